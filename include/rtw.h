@@ -1,0 +1,208 @@
+/*
+ * rtw.h -- C ABI of librtw, the MI355X (gfx950) per-pixel geodesic integrator
+ * for RTWCA scenes.
+ *
+ * librtw replaces the device half of RTWCA's CUDA.jl integrator mode ("5P"):
+ * the camera-ray initialisation, the per-pixel adaptive Dormand-Prince RK45
+ * integration of the geodesic plus radiative transfer, and the spectrum ->
+ * CIE XYZ -> sRGB reduction.  A Julia host binds it through `ccall` as a third
+ * integrator mode; Python binds it through ctypes (rtwca_amd.host).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative RTW_E* code on
+ *    failure; rtw_last_error() gives the message.  Nothing is thrown across the
+ *    ABI.
+ *  - Host buffers are owned by the caller.  A ray's state is
+ *    [t r theta phi k^t k^r k^theta k^phi (I_1/nu^3, tau_1) ... (I_N/nu^3, tau_N)],
+ *    n = 8 + 2 N components; ray matrices are column-major (num_pix, n) exactly
+ *    like long_ray_matrix (main/Iterators/LttM/LttM.jl:141).
+ *  - Pixel p = i + j * x_pix with i the horizontal and j the vertical index of
+ *    the camera grid (main/initialization.jl:213-220).
+ *  - Calls on one context are synchronous (they return after the stream has
+ *    drained, like CUDA.@sync in main/Iterators/5P/5P.jl:205) except
+ *    rtw_render_dev, which only enqueues.  One call in flight per context.
+ */
+#ifndef RTW_H
+#define RTW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTW_ABI_VERSION 1
+
+/* status codes */
+#define RTW_OK 0
+#define RTW_E_ARG (-1)       /* invalid argument */
+#define RTW_E_DEVICE (-2)    /* HIP runtime error */
+#define RTW_E_NOMEM (-3)     /* device allocation failed */
+#define RTW_E_CAMERA (-4)    /* camera on a coordinate singularity (LttM.jl:109-112) */
+#define RTW_E_UNSUPPORTED (-5)
+
+/* Scene kinds.  Each replaces one *.source directory's function set
+ * (metric, Christoffel symbols, vierbein, fluid velocity, emission model,
+ * step cap, termination), selected by id instead of Julia `include`. */
+enum rtw_scene_kind {
+  RTW_SCENE_MINKOWSKI_SPHERE = 1,   /* main/Minkowski_Sphere_Spherical.source */
+  RTW_SCENE_SCHWARZSCHILD_DISK = 2, /* main/Schwarzschild_Simple_Disk_Spherical.source */
+  RTW_SCENE_KERR_DISK = 3,          /* main/Kerr.source (spin a; Powehi geometry at a = 0.9) */
+  RTW_SCENE_KERR_THICK_FF = 4,      /* Kerr + Gaussian thick disk, free-free with van Hoof Gaunt */
+  RTW_SCENE_KERR_SYNCH = 5          /* Kerr + thermal-synchrotron torus and conical jet */
+};
+
+/* RTW_SCENE_REF_F32: evaluate the Kerr family as the reference's Float64 mode
+ * does, i.e. Christoffel symbols in Float32 (main/Kerr.source/source_geometry.jl:196-200),
+ * spin/horizon/ISCO constants in Float32 (:7-37) and Float32(max_dt_scale)
+ * (main/Kerr.source/source_model.jl:93-95). */
+#define RTW_SCENE_REF_F32 1
+
+typedef struct rtw_scene {
+  int32_t kind;          /* rtw_scene_kind */
+  int32_t flags;         /* RTW_SCENE_* */
+  double mass;           /* M (geometric units) */
+  double spin;           /* a */
+  double temperature;    /* K */
+  double absorb;         /* constant a_nu (blackbody scenes) or opacity scale kappa (free-free) */
+  double density;        /* density scale rho_s (free-free) / electron density scale (synchrotron) */
+  double r_in, r_out;    /* radial extent of the emitting matter */
+  double th_lo, th_hi;   /* theta wedge of the thin disks */
+  double h_over_r;       /* thick disk scale height H / r */
+  double z_cut;          /* thick disk truncation |z| <= z_cut * H */
+  double r_escape;       /* escape radius (100 in every reference scene) */
+  double bfield;         /* synchrotron: B in Gauss */
+  double theta_e;        /* synchrotron: electron temperature k T_e / (m_e c^2) */
+  double jet_half_angle; /* synchrotron: jet half opening angle (rad) */
+  double r_jet;          /* synchrotron: jet base radius */
+  double torus_r_max;    /* synchrotron: torus pressure maximum radius */
+} rtw_scene;
+
+/* Camera, the arguments of initialize_camera (main/initialization.jl:190-192). */
+typedef struct rtw_camera {
+  double pos[4];    /* camera 4-position */
+  double dir[2];    /* direction angles of the camera velocity w.r.t. the local tetrad */
+  double beta;      /* speed w.r.t. the tetrad, in c */
+  double point[3];  /* roll, pitch, yaw of the camera w.r.t. its velocity */
+  int32_t x_pix;    /* horizontal pixels; y_pix = round(x_pix * tan(fov_v/2) / tan(fov_h/2)) */
+  int32_t reserved;
+  double fov_hor;   /* degrees */
+  double fov_vert;  /* degrees */
+} rtw_camera;
+
+enum rtw_precision {
+  RTW_PREC_FP64 = 0,  /* every stage in FP64 (the plain method) */
+  RTW_PREC_MIXED = 1  /* FP32 stage evaluation, FP64 state/error/controller, FP64 steps near the hole */
+};
+enum rtw_schedule {
+  RTW_SCHED_GRID = 0,       /* one launch slot per ray */
+  RTW_SCHED_PERSISTENT = 1  /* persistent lane groups pulling rays from a device work queue */
+};
+
+/* The integrator arguments of gen_image (main/Iterators/LttM/LttM.jl:104-107),
+ * with separate absolute and relative tolerances. */
+typedef struct rtw_integrator {
+  double rtol;
+  double atol;
+  double max_dt_scale;
+  double max_steps;
+  int32_t precision;      /* rtw_precision */
+  int32_t schedule;       /* rtw_schedule */
+  double fp64_radius;     /* MIXED: steps starting at r < fp64_radius run in FP64 */
+  int32_t lanes_per_ray;  /* 0 = library default */
+  int32_t reserved;
+} rtw_integrator;
+
+/* Termination class per pixel (the reference stops without recording why;
+ * the rule is: horizon, else escape, else optically thick in every colour,
+ * else boundary = max_steps, phi winding or a non-finite state). */
+enum rtw_class {
+  RTW_CLASS_HORIZON = 1,
+  RTW_CLASS_ESCAPE = 2,
+  RTW_CLASS_OPAQUE = 3,
+  RTW_CLASS_BOUNDARY = 4
+};
+
+typedef struct rtw_stats {
+  double kernel_ms;         /* device time of the integration kernel(s) */
+  double total_ms;          /* wall time of the call */
+  int64_t rays;
+  int64_t steps_attempted;  /* sum over rays of attempted RK45 steps (LttM.jl:43-46) */
+  int64_t steps_accepted;
+  int64_t steps_fp64;       /* MIXED: attempted steps that ran in FP64 */
+  int32_t lanes_per_ray;
+  int32_t reserved;
+} rtw_stats;
+
+typedef struct rtw_ctx rtw_ctx;
+
+/* Library ABI version (RTW_ABI_VERSION).  No reference counterpart. */
+int rtw_version(void);
+
+/* Create / destroy a context bound to one HIP device.  The context owns the
+ * device buffers, the staged van Hoof Gaunt table and the stream.
+ * Replaces the CUDA.jl device set-up implied by `using CUDA` (main/Iterators/5P/5P.jl:5-9). */
+int rtw_context_create(int32_t device, rtw_ctx** out);
+void rtw_context_destroy(rtw_ctx* ctx);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * rtw_context_create when ctx is NULL). */
+const char* rtw_last_error(const rtw_ctx* ctx);
+
+/* Load the 146 x 81 Gaunt-factor table (gauntff_merged_Z01.dat) that the
+ * free-free scene interpolates.  Replaces the CSV load at
+ * main/emission_functions.jl:15. */
+int rtw_load_gaunt(rtw_ctx* ctx, const char* path);
+
+/* Render n_cams camera poses of one scene.
+ * Replaces the integration part of gen_image (main/Iterators/5P/5P.jl:145-215 and
+ * main/Iterators/LttM/LttM.jl:114-179): camera rays (initialize_camera,
+ * main/initialization.jl:190-331) are generated on the device unless init_rays is
+ * given; every ray is integrated to termination; the per-pixel CIE XYZ of
+ * calc_xyY (main/color_theory.jl:96-102) is reduced on the device.
+ *   lambda_nm   n_lambda wavelengths in nm (the reference's `colors`)
+ *   init_rays   NULL, or column-major (num_pix, 8) FP64 initial positions and
+ *               momenta for a single pose (spectral components start at 0)
+ *   out_xyz     3 * num_pix doubles, X,Y,Z per pixel (pose-major)
+ *   out_class   num_pix bytes, rtw_class per pixel (NULL allowed)
+ *   out_steps   2 * num_pix int32: attempted, accepted steps per pixel (NULL allowed)
+ *   out_rays    column-major (num_pix, 8 + 2 n_lambda) final states (returnrays=true;
+ *               NULL allowed)
+ *   stats       NULL allowed
+ * num_pix = n_cams * x_pix * y_pix; every pose must share x_pix and the FOVs. */
+int rtw_render(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams,
+               const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
+               const double* init_rays, double* out_xyz, uint8_t* out_class, int32_t* out_steps,
+               double* out_rays, rtw_stats* stats);
+
+/* As rtw_render, but the outputs are device pointers and the work is only
+ * enqueued on `stream` (a hipStream_t; NULL = the context's stream).  For
+ * callers that keep images resident in HBM (benchmarks, torch interop).
+ * d_steps may be NULL.  No reference counterpart beyond rtw_render's. */
+int rtw_render_dev(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams,
+                   const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
+                   double* d_xyz, uint8_t* d_class, int32_t* d_steps, void* stream);
+
+/* Brightness normalisation and conversion of one pose's XYZ to 16-bit sRGB,
+ * returned as the (y_pix, x_pix) row-major image with the reference's
+ * orientation transpose(reverse(img, dims=2)).  Replaces LttM.jl:182-208
+ * (maxfinite, scaleminmax, convert(RGB{N0f16}, xyY)).  out_rgb16 holds
+ * 3 * x_pix * y_pix values (R,G,B interleaved).  Returns RTW_OK also for a
+ * blank image (all black, LttM.jl:184-191). */
+int rtw_xyz_to_srgb16(rtw_ctx* ctx, const double* xyz, int32_t x_pix, int32_t y_pix, uint16_t* out_rgb16);
+
+/* Gradient of L = sum over pixels and X,Y,Z of dL_dxyz * XYZ with respect to
+ * (spin, camera inclination theta, disk temperature, density scale) for the
+ * free-free thick-disk scene, by a hand-written discrete adjoint of the RK45
+ * geodesic + transfer integration with checkpointed recomputation of the stage
+ * states; the accepted step sizes are held fixed.  out_xyz (NULL allowed)
+ * receives the forward image; out_grad receives 4 doubles.
+ * No counterpart in the reference (it has no differentiation). */
+int rtw_render_adjoint(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cam,
+                       const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
+                       const double* dL_dxyz, double* out_xyz, double* out_grad, rtw_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTW_H */

@@ -1,0 +1,239 @@
+"""ctypes loader of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of RTWCA's Float64 integrator (rtw_oracle.hpp).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it,
+and only as the checker / the timed CPU baseline, never inside the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from rtwca_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+GAUNT_PATH = os.path.join(os.path.dirname(HERE), "rtwca_amd", "data", "gauntff_merged_Z01.dat")
+
+_lib = None
+
+D = C.c_double
+PD = C.POINTER(C.c_double)
+P = C.c_void_p
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.rtwo_gaunt.restype = C.c_float
+        L.rtwo_gaunt.argtypes = [D, D]
+        for name in ("rtwo_planck", "rtwo_circ_beta"):
+            getattr(L, name).restype = D
+        L.rtwo_planck.argtypes = [D, D]
+        L.rtwo_isco_ref.restype = D
+        L.rtwo_isco_ref.argtypes = [D, D, C.c_int]
+        L.rtwo_circ_beta.argtypes = [D, D, D]
+        L.rtwo_gam2_factor.restype = D
+        L.rtwo_pad_max_dt.restype = D
+        L.rtwo_pad_max_dt.argtypes = [P, P, D]
+        L.rtwo_near_singularity.argtypes = [P, P, D, D, P]
+        L.rtwo_terminate.argtypes = [P, P, C.c_int, P, D]
+        L.rtwo_classify.argtypes = [P, P, C.c_int, P]
+        L.rtwo_cie_xyz_fit.argtypes = [D, P]
+        L.rtwo_ode_trace.argtypes = [D, D, D, D, D, D, C.c_int, P, P, P, P, P]
+        L.rtwo_render.argtypes = [P, P, P, P, C.c_int, C.c_int, P, C.c_int64, P, P, P, P, P, P]
+        L.rtwo_count_ops.argtypes = [P, P, P, P, C.c_int, P, C.c_int64, P]
+        L.rtwo_derivative.argtypes = [P, P, P, C.c_int, P, P]
+        L.rtwo_ray_xyz.argtypes = [P, P, C.c_int, P]
+        rc = L.rtwo_load_gaunt(GAUNT_PATH.encode())
+        if rc != 0:
+            raise RuntimeError("oracle: cannot load Gaunt table (%d)" % rc)
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _r(x):
+    return C.cast(C.byref(x), C.c_void_p)
+
+
+def gaunt(gam2, u):
+    return lib().rtwo_gaunt(gam2, u)
+
+
+def planck(T, f):
+    return lib().rtwo_planck(T, f)
+
+
+def isco_ref(a, M=1.0, f32=False):
+    return lib().rtwo_isco_ref(a, M, int(f32))
+
+
+def scene_derived(scene):
+    out = np.zeros(3)
+    lib().rtwo_scene_derived(_r(scene), _p(out))
+    return {"r_s": out[0], "r_ev": out[1], "spin": out[2]}
+
+
+def metric(scene, x):
+    out = np.zeros(16)
+    lib().rtwo_metric(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(out))
+    return out.reshape(4, 4)
+
+
+def christoffel(scene, x):
+    out = np.zeros(64)
+    lib().rtwo_christoffel(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(out))
+    return out.reshape(4, 4, 4)
+
+
+def source_velocity(scene, x):
+    out = np.zeros(4)
+    lib().rtwo_source_velocity(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(out))
+    return out
+
+
+def pad_max_dt(scene, ray8, mds):
+    return lib().rtwo_pad_max_dt(_r(scene), _p(np.ascontiguousarray(ray8, dtype=np.float64)), mds)
+
+
+def near_singularity(scene, ray8, h, atol):
+    oh = C.c_double(0)
+    r = lib().rtwo_near_singularity(_r(scene), _p(np.ascontiguousarray(ray8, dtype=np.float64)), h, atol,
+                                    C.cast(C.byref(oh), C.c_void_p))
+    return bool(r), oh.value
+
+
+def keepinbounds(scene, ray8):
+    a = np.array(ray8, dtype=np.float64)
+    lib().rtwo_keepinbounds(_r(scene), _p(a))
+    return a
+
+
+def derivative(scene, integ, lam, ray):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    ray = np.ascontiguousarray(ray, dtype=np.float64)
+    out = np.zeros_like(ray)
+    lib().rtwo_derivative(_r(scene), _r(integ), _p(lam), len(lam), _p(ray), _p(out))
+    return out
+
+
+def terminate(scene, integ, nlam, ray, step_count):
+    ray = np.ascontiguousarray(ray, dtype=np.float64)
+    return bool(lib().rtwo_terminate(_r(scene), _r(integ), nlam, _p(ray), float(step_count)))
+
+
+def classify(scene, integ, nlam, ray):
+    ray = np.ascontiguousarray(ray, dtype=np.float64)
+    return lib().rtwo_classify(_r(scene), _r(integ), nlam, _p(ray))
+
+
+def dp_tableau():
+    out = np.zeros(72)
+    lib().rtwo_dp_tableau(_p(out))
+    return out.reshape(9, 8)
+
+
+def cie_weights(lam):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    W = np.zeros((3, len(lam)))
+    lib().rtwo_cie_weights(_p(lam), len(lam), _p(W[0]), _p(W[1]), _p(W[2]))
+    return W
+
+
+def cie_fit(wl):
+    out = np.zeros(3)
+    lib().rtwo_cie_xyz_fit(float(wl), _p(out))
+    return out
+
+
+def ray_xyz(ray, lam):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    ray = np.ascontiguousarray(ray, dtype=np.float64)
+    out = np.zeros(3)
+    lib().rtwo_ray_xyz(_p(ray), _p(lam), len(lam), _p(out))
+    return out
+
+
+def xyz_to_xyY(xyz):
+    out = np.zeros(3)
+    lib().rtwo_xyz_to_xyY(_p(np.ascontiguousarray(xyz, dtype=np.float64)), _p(out))
+    return out
+
+
+def xyY_to_srgb16(xyY):
+    out = np.zeros(3, dtype=np.uint16)
+    lib().rtwo_xyY_to_srgb16(_p(np.ascontiguousarray(xyY, dtype=np.float64)), _p(out))
+    return out
+
+
+def camera_y_pix(cam):
+    return lib().rtwo_camera_y_pix(_r(cam))
+
+
+def camera_matrix(scene, cam):
+    out = np.zeros(16)
+    lib().rtwo_camera_matrix(_r(scene), _r(cam), _p(out))
+    return out.reshape(4, 4)
+
+
+def camera_rays(scene, cam):
+    ny = camera_y_pix(cam)
+    npix = cam.x_pix * ny
+    out = np.zeros(npix * 8)
+    lib().rtwo_camera_rays(_r(scene), _r(cam), _p(out))
+    return out.reshape(8, npix).T.copy()  # (npix, 8)
+
+
+def ode_trace(lam, y0, h0, max_dt, atol, rtol, nsteps):
+    y = np.zeros(nsteps)
+    h = np.zeros(nsteps)
+    rej = np.zeros(nsteps, dtype=np.int32)
+    k1 = np.zeros(nsteps)
+    err = np.zeros(nsteps)
+    lib().rtwo_ode_trace(lam, y0, h0, max_dt, atol, rtol, nsteps, _p(y), _p(h), _p(rej), _p(k1), _p(err))
+    return y, h, rej.astype(bool), k1, err
+
+
+def render(scene, cam, integ, lam, nthreads=1, subset=None, init_rays=None, want_rays=False):
+    """Integrate the camera's rays (all, or the ray indices in `subset`).
+    Returns dict(xyz (npix,3), cls (npix,), steps (npix,2), rays (npix,n) or None, seconds)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    ny = camera_y_pix(cam)
+    npix = cam.x_pix * ny
+    n = 8 + 2 * len(lam)
+    xyz = np.zeros((npix, 3))
+    cls = np.zeros(npix, dtype=np.uint8)
+    steps = np.zeros((npix, 2), dtype=np.int64)
+    rays = np.zeros(npix * n) if want_rays else None
+    sub = None if subset is None else np.ascontiguousarray(subset, dtype=np.int64)
+    ir = None
+    if init_rays is not None:
+        ir = np.ascontiguousarray(np.asarray(init_rays, dtype=np.float64).T.reshape(-1))
+    secs = C.c_double(0)
+    rc = lib().rtwo_render(_r(scene), _r(cam), _r(integ), _p(lam), len(lam), int(nthreads), _p(sub),
+                           0 if sub is None else len(sub), _p(ir), _p(xyz), _p(cls), _p(steps), _p(rays),
+                           C.cast(C.byref(secs), C.c_void_p))
+    if rc != 0:
+        raise RuntimeError("oracle render failed: %d" % rc)
+    return {"xyz": xyz, "cls": cls, "steps": steps,
+            "rays": None if rays is None else rays.reshape(n, npix).T.copy(), "seconds": secs.value}
+
+
+def count_ops(scene, cam, integ, lam, subset):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    sub = np.ascontiguousarray(subset, dtype=np.int64)
+    out = np.zeros(3)
+    lib().rtwo_count_ops(_r(scene), _r(cam), _r(integ), _p(lam), len(lam), _p(sub), len(sub), _p(out))
+    return {"arith": out[0], "trans": out[1], "steps": out[2]}

@@ -1,0 +1,960 @@
+// rtw_oracle.hpp -- CPU restatement of RTWCA's Float64 (LttM) integrator path.
+//
+// TEST INFRASTRUCTURE ONLY.  This header is the oracle that the HIP kernels of
+// rtwca_amd are checked against.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it (through liboracle.so); the product
+// library never includes, links or calls it.
+//
+// Every function follows the reference line by line (citations are
+// path:line into the RTWCA repository), including its quirks:
+//   * FSAL by average: k1 of the next step is sum(b_i k_i), not f(y_{n+1})
+//     (main/Iterators/LttM/integrators_LttM.jl:147-150,199,219);
+//   * Kerr Christoffel symbols in Float32 when REF_F32 is set
+//     (main/Kerr.source/source_geometry.jl:196-200);
+//   * the "ISCO" formula with (1 +- chi)^(3/2) (Kerr.source/source_geometry.jl:23);
+//   * the van Hoof Gaunt lookup offset by one cell and its weight-sum-2 edge
+//     (main/emission_functions.jl:61-72);
+//   * Julia's floored mod in keepinbounds! (Kerr.source/source_geometry.jl:290);
+//   * the Euler fallback near the poles (integrators_LttM.jl:62-105);
+//   * no fused multiply-add: build with -ffp-contract=off (Julia does not fuse).
+//
+// The integrator core is templated on the scalar type R so the same code runs
+// in double (the oracle proper) and in `Cnt`, a counting scalar that measures
+// the arithmetic work of one attempted RK45 step (add/sub/mul/div/sqrt = 1,
+// each transcendental call = 1).
+//
+// Scenes without a reference counterpart (the Kerr thick free-free disk and the
+// thermal-synchrotron torus + jet) are defined here first; the kernel files
+// restate the same formulas.
+//
+// Parity status of each part and the checks that pin it are listed in the
+// header of tests/test_oracle.py.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <utility>
+#include <vector>
+#include <algorithm>
+#include <limits>
+
+namespace rtwo {
+
+// ---------------------------------------------------------------------------
+// Counting scalar
+// ---------------------------------------------------------------------------
+struct OpCount { uint64_t arith = 0, trans = 0; };
+inline OpCount& op_count() { static thread_local OpCount c; return c; }
+
+struct Cnt {
+  double v;
+  Cnt() : v(0) {}
+  Cnt(double x) : v(x) {}
+  explicit operator double() const { return v; }
+};
+inline Cnt ar(double x) { op_count().arith++; return Cnt(x); }
+inline Cnt tr(double x) { op_count().trans++; return Cnt(x); }
+inline Cnt operator+(Cnt a, Cnt b) { return ar(a.v + b.v); }
+inline Cnt operator-(Cnt a, Cnt b) { return ar(a.v - b.v); }
+inline Cnt operator*(Cnt a, Cnt b) { return ar(a.v * b.v); }
+inline Cnt operator/(Cnt a, Cnt b) { return ar(a.v / b.v); }
+inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+inline Cnt& operator+=(Cnt& a, Cnt b) { a = a + b; return a; }
+inline Cnt& operator-=(Cnt& a, Cnt b) { a = a - b; return a; }
+inline Cnt& operator*=(Cnt& a, Cnt b) { a = a * b; return a; }
+inline Cnt& operator/=(Cnt& a, Cnt b) { a = a / b; return a; }
+inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
+inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+inline bool operator<=(Cnt a, Cnt b) { return a.v <= b.v; }
+inline bool operator>=(Cnt a, Cnt b) { return a.v >= b.v; }
+inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
+inline bool operator!=(Cnt a, Cnt b) { return a.v != b.v; }
+inline Cnt sqrt(Cnt a) { return ar(std::sqrt(a.v)); }
+inline Cnt exp(Cnt a) { return tr(std::exp(a.v)); }
+inline Cnt log(Cnt a) { return tr(std::log(a.v)); }
+inline Cnt log10(Cnt a) { return tr(std::log10(a.v)); }
+inline Cnt sin(Cnt a) { return tr(std::sin(a.v)); }
+inline Cnt cos(Cnt a) { return tr(std::cos(a.v)); }
+inline Cnt tan(Cnt a) { return tr(std::tan(a.v)); }
+inline Cnt pow(Cnt a, Cnt b) { return tr(std::pow(a.v, b.v)); }
+inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
+inline Cnt fmod(Cnt a, Cnt b) { return ar(std::fmod(a.v, b.v)); }
+inline bool isfinite(Cnt a) { return std::isfinite(a.v); }
+inline double val(Cnt a) { return a.v; }
+inline double val(double a) { return a; }
+
+using std::sqrt; using std::exp; using std::log; using std::log10; using std::sin;
+using std::cos; using std::tan; using std::pow; using std::fabs; using std::fmod;
+using std::isfinite;
+
+template <class R> inline R rmin(R a, R b) { return (b < a) ? b : a; }   // Julia min (no NaN case needed)
+template <class R> inline R rmax(R a, R b) { return (a < b) ? b : a; }
+template <class R> inline R sq(R x) { return x * x; }        // Julia literal x^2
+template <class R> inline R cube(R x) { return x * x * x; }  // Julia literal x^3 = (x*x)*x
+
+// ---------------------------------------------------------------------------
+// Constants (main/64BitConstants.jl:13-37)
+// ---------------------------------------------------------------------------
+constexpr double PI = 3.141592653589793;                 // initialization.jl:62 (Float64 pi)
+constexpr double H_PLANCK = 6.62607015e-34;              // :13
+constexpr double C_LIGHT = 299792458.0;                  // :17 (Int, exact in double)
+constexpr double K_B = 1.380649e-23;                     // :19
+constexpr double NO_DIV_ZERO = 1e-24;                    // :21
+constexpr double EMISSION_SCALE = 2 * H_PLANCK * C_LIGHT * 1e27;  // :23  ((2h)c)1e27
+constexpr double F_FACTOR = H_PLANCK * C_LIGHT * 1e9 / K_B;       // :25  ((hc)1e9)/k_B
+// Ry / k_B in K: gam2 = Ry/(k_B T) (comment at emission_functions.jl:56)
+constexpr double RYDBERG_J = 2.1798723611035e-18;
+constexpr double GAM2_FACTOR = RYDBERG_J / K_B;
+
+// Dormand-Prince tableau (64BitConstants.jl:28-37): row s (1..9), col 1 = c_s.
+struct DP {
+  double a[10][9];
+  DP() {
+    std::memset(a, 0, sizeof(a));
+    double t[9][8] = {
+        {0, 0, 0, 0, 0, 0, 0, 0},
+        {1.0 / 5, 1.0 / 5, 0, 0, 0, 0, 0, 0},
+        {3.0 / 10, 3.0 / 40, 9.0 / 40, 0, 0, 0, 0, 0},
+        {4.0 / 5, 44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0, 0},
+        {8.0 / 9, 19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0, 0},
+        {1, 9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0, 0},
+        {1, 35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84, 0},
+        {0, 35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84, 0},
+        {0, 5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100, 1.0 / 40}};
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 8; ++j) a[i + 1][j + 1] = t[i][j];  // 1-based like Julia
+  }
+};
+inline const DP& dp() { static const DP d; return d; }
+
+// ---------------------------------------------------------------------------
+// Scenes
+// ---------------------------------------------------------------------------
+enum SceneKind {
+  MINKOWSKI_SPHERE = 1,   // Minkowski_Sphere_Spherical.source
+  SCHWARZSCHILD_DISK = 2, // Schwarzschild_Simple_Disk_Spherical.source
+  KERR_DISK = 3,          // Kerr.source (and Powehi geometry with a = 0.9)
+  KERR_THICK_FF = 4,      // new: Kerr geometry + Gaussian thick disk, free-free with van Hoof Gaunt
+  KERR_SYNCH = 5,         // new: Kerr geometry + thermal-synchrotron torus and conical jet
+};
+enum SceneFlags {
+  REF_F32 = 1,  // Kerr family: Float32 Christoffel, horizon/ISCO and max_dt_scale as the reference
+};
+
+struct Scene {
+  int kind = 0;
+  int flags = 0;
+  double mass = 1.0;      // M
+  double spin = 0.0;      // a (Kerr family)
+  double temperature = 5778.0;
+  double absorb = 1.0;    // constant a_nu (blackbody scenes) / opacity kappa (free-free)
+  double density = 1.0;   // density scale rho_s (free-free), n_e scale (synchrotron)
+  double r_in = 0, r_out = 0;        // matter radial extent
+  double th_lo = 0, th_hi = 0;       // wedge in theta
+  double h_over_r = 0.1, z_cut = 4;  // thick disk: H = h_over_r * r, |z| <= z_cut * H
+  double r_escape = 100.0;           // calc_terminate "ray[2] > 100"
+  double bfield = 10.0;              // synchrotron: B in Gauss
+  double theta_e = 10.0;             // synchrotron: k T_e / (m_e c^2)
+  double jet_half_angle = 0.0;       // synchrotron: jet opening half-angle (rad)
+  double r_jet = 0.0;                // synchrotron: jet base radius
+  double torus_r_max = 12.0;         // synchrotron: torus pressure maximum
+  // derived by scene_init
+  double r_s = 2.0, r_ev = 0.0;
+};
+
+// calc_orb_limits prograde ISCO exactly as Kerr.source/source_geometry.jl:20-26 (Q5:
+// uses (1 +- chi)^(3/2)).  With f32 the inputs are Float32 as in the reference.
+inline double kerr_isco_ref(double a, double M, bool f32) {
+  if (f32) {
+    float af = (float)a, Mf = (float)M, rs = 2.0f * Mf;
+    float chi = 2.0f * af / rs;
+    double Z1 = 1.0 + std::pow((double)(1.0f - chi * chi), 1.0 / 3.0) *
+                          (std::pow((double)(1.0f + chi), 1.5) + std::pow((double)(1.0f - chi), 1.5));
+    // 3*chi^2 is Float32 in Julia and is promoted when added to Z1^2
+    double Z2 = std::sqrt((double)(3.0f * (chi * chi)) + Z1 * Z1);
+    return (double)(float)((double)Mf * (3.0 + Z2 - std::sqrt((3.0 - Z1) * (3.0 + Z1 + 2.0 * Z2))));
+  }
+  double rs = 2 * M, chi = 2 * a / rs;
+  double Z1 = 1 + std::pow(1 - chi * chi, 1.0 / 3.0) * (std::pow(1 + chi, 1.5) + std::pow(1 - chi, 1.5));
+  double Z2 = std::sqrt(3 * (chi * chi) + Z1 * Z1);
+  return M * (3 + Z2 - std::sqrt((3 - Z1) * (3 + Z1 + 2 * Z2)));
+}
+
+// calc_circ_geod_beta (Kerr.source/source_geometry.jl:44-49)
+inline double kerr_circ_beta(double r, double a, double M) {
+  double rs = 2 * M;
+  double D = r * r - r * rs + a * a;
+  return std::sqrt(M) * (r * r - 2 * a * std::sqrt(M * r) + a * a) / (std::sqrt(D) * (std::pow(r, 1.5) + a * std::sqrt(M)));
+}
+
+inline void scene_init(Scene& s) {
+  s.r_s = 2 * s.mass;
+  if (s.kind == SCHWARZSCHILD_DISK) {
+    s.r_ev = s.r_s;  // Schwarzschild_...source_model.jl:358 terminates at (1+10 rtol) r_s
+  } else if (s.kind >= KERR_DISK) {
+    if (s.flags & REF_F32) {  // Kerr.source/source_geometry.jl:7-17 in Float32
+      float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
+      s.r_ev = (double)(rs / 2 + std::sqrt((rs * rs / 4) - a * a));
+      s.spin = (double)a;
+      s.r_s = (double)rs;
+    } else {
+      s.r_ev = s.r_s / 2 + std::sqrt((s.r_s * s.r_s / 4) - s.spin * s.spin);
+    }
+  } else {
+    s.r_ev = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+// calc_lower_metric! : Minkowski source_geometry.jl:10-16, Schwarzschild :15-21, Kerr :55-75.
+template <class R> void lower_metric(const Scene& s, const R* x, R g[4][4]) {
+  if (s.kind == MINKOWSKI_SPHERE) {
+    g[0][0] = R(-1.0);
+    g[1][1] = R(1.0);
+    g[2][2] = sq(x[1]);
+    g[3][3] = sq(x[1] * sin(x[2]));
+  } else if (s.kind == SCHWARZSCHILD_DISK) {
+    R rs(s.r_s);
+    g[0][0] = -(R(1.0) - rs / x[1]);
+    g[1][1] = R(1.0) / (R(1.0) - rs / x[1]);
+    g[2][2] = sq(x[1]);
+    g[3][3] = sq(x[1] * sin(x[2]));
+  } else {
+    R a(s.spin), rs(s.r_s);
+    R Sig = sq(x[1]) + sq(a * cos(x[2]));
+    R Del = sq(x[1]) - x[1] * rs + sq(a);
+    R A = (sq(x[1]) + sq(a)) * Sig + rs * x[1] * sq(a * sin(x[2]));
+    g[0][0] = -(R(1.0) - rs * x[1] / Sig);
+    g[0][3] = -rs * a * x[1] * sq(sin(x[2])) / Sig;
+    g[3][0] = g[0][3];
+    g[1][1] = Sig / Del;
+    g[2][2] = Sig;
+    g[3][3] = (A / Sig) * sq(sin(x[2]));
+  }
+}
+
+// Kerr Christoffel symbols (Kerr.source/source_geometry.jl:195-243), in scalar T.
+template <class T> void kerr_christoffel_T(T r, T th, T a, T rs, T G[4][4][4]) {
+  T Sig = sq(r) + sq(a * cos(th));
+  T Del = sq(r) - r * rs + sq(a);
+  T A = (sq(r) + sq(a)) * Sig + rs * r * sq(a * sin(th));
+  T two(2), three(3);
+  // t
+  G[0][0][1] = rs * (sq(r) + sq(a)) * (sq(r) - sq(a * cos(th))) / (two * sq(Sig) * Del);
+  G[0][1][0] = G[0][0][1];
+  G[0][0][2] = -rs * sq(a) * r * sin(two * th) / (two * sq(Sig));
+  G[0][2][0] = G[0][0][2];
+  G[0][1][3] = rs * a * sq(sin(th)) * (sq(a * cos(th)) * (sq(a) - sq(r)) - sq(r) * (sq(a) + three * sq(r))) /
+               (two * sq(Sig) * Del);
+  G[0][3][1] = G[0][1][3];
+  G[0][2][3] = rs * cube(a) * r * cube(sin(th)) * cos(th) / sq(Sig);
+  G[0][3][2] = G[0][2][3];
+  // r
+  G[1][0][0] = rs * Del * (sq(r) - sq(a) * sq(cos(th))) / (two * cube(Sig));
+  G[1][0][3] = -Del * rs * a * sq(sin(th)) * (sq(r) - sq(a * cos(th))) / (two * cube(Sig));
+  G[1][3][0] = G[1][0][3];
+  G[1][1][1] = (two * r * sq(a * sin(th)) - rs * (sq(r) - sq(a * cos(th)))) / (two * Sig * Del);
+  G[1][1][2] = -sq(a) * sin(two * th) / (two * Sig);
+  G[1][2][1] = G[1][1][2];
+  G[1][2][2] = -r * Del / Sig;
+  G[1][3][3] = Del * sq(sin(th)) * (-two * r * sq(Sig) + rs * sq(a * sin(th)) * (sq(r) - sq(a * cos(th)))) /
+               (two * cube(Sig));
+  // theta
+  G[2][0][0] = -rs * sq(a) * r * sin(two * th) / (two * cube(Sig));
+  G[2][0][3] = rs * a * r * (sq(r) + sq(a)) * sin(two * th) / (two * cube(Sig));
+  G[2][3][0] = G[2][0][3];
+  G[2][1][1] = sq(a) * sin(two * th) / (two * Sig * Del);
+  G[2][1][2] = r / Sig;
+  G[2][2][1] = G[2][1][2];
+  G[2][2][2] = -sq(a) * sin(two * th) / (two * Sig);
+  G[2][3][3] = -sin(two * th) * (A * Sig + (sq(r) + sq(a)) * rs * r * sq(a * sin(th))) / (two * cube(Sig));
+  // phi
+  G[3][0][1] = rs * a * (sq(r) - sq(a * cos(th))) / (two * sq(Sig) * Del);
+  G[3][1][0] = G[3][0][1];
+  G[3][0][2] = -rs * a * r * (T(1) / tan(th)) / sq(Sig);
+  G[3][2][0] = G[3][0][2];
+  G[3][1][3] = (two * r * sq(Sig) + rs * (sq(sq(a) * sin(two * th) / two) - sq(r) * (Sig + sq(r) + sq(a)))) /
+               (two * sq(Sig) * Del);
+  G[3][3][1] = (two * r * sq(Sig) + rs * (sq(sq(a) * sin(two * th) / two) - sq(r) * (Sig + sq(r) + sq(a)))) /
+               (two * sq(Sig) * Del);
+  G[3][2][3] = (T(1) / tan(th)) * (sq(Sig) + rs * r * sq(a * sin(th))) / sq(Sig);
+  G[3][3][2] = G[3][2][3];
+}
+
+// calc_christoffel_udd!: Minkowski source_geometry.jl:93-108, Schwarzschild :109-128,
+// Kerr :195-243.  G must be zero-initialised once per ray (LttM.jl:21); only the
+// listed entries are written.
+template <class R> void christoffel(const Scene& s, const R* x, R G[4][4][4]) {
+  if (s.kind == MINKOWSKI_SPHERE) {
+    R d(NO_DIV_ZERO);
+    G[1][2][2] = -x[1];
+    G[1][3][3] = -x[1] * sq(sin(x[2]));
+    G[2][1][2] = R(1) / (x[1] + d);
+    G[2][2][1] = G[2][1][2];
+    G[2][3][3] = -sin(x[2]) * cos(x[2]);
+    G[3][1][3] = R(1) / (x[1] + d);
+    G[3][3][1] = G[3][1][3];
+    G[3][2][3] = R(1) / tan(x[2]);
+    G[3][3][2] = G[3][2][3];
+  } else if (s.kind == SCHWARZSCHILD_DISK) {
+    R rs(s.r_s), two(2);
+    G[0][0][1] = rs / (two * x[1] * (x[1] - rs));
+    G[0][1][0] = G[0][0][1];
+    G[1][0][0] = rs * (x[1] - rs) / (two * cube(x[1]));
+    G[1][1][1] = -rs / (two * x[1] * (x[1] - rs));
+    G[1][2][2] = -(x[1] - rs);
+    G[1][3][3] = -(x[1] - rs) * sq(sin(x[2]));
+    G[2][1][2] = R(1) / x[1];
+    G[2][2][1] = G[2][1][2];
+    G[2][3][3] = -sin(x[2]) * cos(x[2]);
+    G[3][1][3] = R(1) / x[1];
+    G[3][3][1] = G[3][1][3];
+    G[3][2][3] = R(1) / tan(x[2]);
+    G[3][3][2] = G[3][2][3];
+  } else if (s.flags & REF_F32) {
+    // Q3: r and theta cast to Float32 and evaluated in Float32 (Kerr.source:196-197)
+    float Gf[4][4][4];
+    std::memset(Gf, 0, sizeof(Gf));
+    kerr_christoffel_T<float>((float)val(x[1]), (float)val(x[2]), (float)s.spin, (float)s.r_s, Gf);
+    // the Kerr routine writes the same 40 entries every call; the rest stay 0
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) G[i][j][k] = R((double)Gf[i][j][k]);
+  } else {
+    kerr_christoffel_T<R>(x[1], x[2], R(s.spin), R(s.r_s), G);
+  }
+}
+
+// near_singularity: Minkowski source_geometry.jl:114-132, Schwarzschild :134-154, Kerr :249-269.
+template <class R> std::pair<bool, R> near_singularity(const Scene& s, const R* ray, R h, double atol2, double atol3) {
+  R d(NO_DIV_ZERO), pi(PI);
+  R dth = h * ray[6];
+  R th_new = ray[2] + dth;
+  R dr = h * ray[5];
+  R new_r = ray[1] + dr;
+  if (s.kind == MINKOWSKI_SPHERE) {
+    if (fabs(th_new) <= R(10) * R(atol3) / fabs(new_r + d))
+      return {true, R(2) * fabs(ray[2]) / (fabs(ray[6] * R(37)) + d)};
+    else if (fabs(th_new - pi) <= R(10) * R(atol3) / fabs(new_r + d))
+      return {true, R(2) * fabs((ray[2] - pi) / (fabs(ray[6] * R(37)) + d))};
+    else if (fabs(new_r) <= R(100) * R(atol2))
+      return {true, R(2) * R(atol2) / (fabs(ray[5] * R(37)) + d)};
+    return {false, h};
+  }
+  if (fabs(th_new) <= R(atol3)) return {true, R(2) * fabs(ray[2]) / (fabs(ray[6] * R(37)) + d)};
+  if (fabs(th_new - pi) <= R(atol3)) return {true, R(2) * fabs((ray[2] - pi) / (fabs(ray[6] * R(37)) + d))};
+  return {false, h};
+}
+
+// Julia mod(x, y) for floats: r = rem(x,y); r == 0 -> copysign(0,y); sign mismatch -> r + y.
+template <class R> R julia_mod(R x, R y) {
+  R r = fmod(x, y);
+  if (val(r) == 0.0) return R(std::copysign(0.0, val(y)));
+  if ((val(r) > 0) != (val(y) > 0)) return r + y;
+  return r;
+}
+
+// keepinbounds!: Minkowski source_geometry.jl:138-163 (with the r <= 0 reflection),
+// Schwarzschild :160-186, Kerr :275-301.
+template <class R> void keepinbounds(const Scene& s, R* ray) {
+  R pi(PI), twopi(2 * PI);
+  if (s.kind == MINKOWSKI_SPHERE && ray[1] <= R(0)) {
+    ray[1] = -ray[1] + R(NO_DIV_ZERO);
+    ray[5] = -ray[5];
+    ray[2] -= pi;
+    ray[6] = -ray[6];
+  }
+  if (ray[2] < R(0)) {
+    ray[2] = -ray[2];
+    ray[3] -= pi;
+    ray[6] = -ray[6];
+  }
+  ray[2] = julia_mod(ray[2], twopi);
+  if (ray[2] >= pi) {
+    ray[2] = twopi - ray[2];
+    ray[3] += pi;
+    ray[6] = -ray[6];
+  }
+}
+
+// pad_max_dt: Minkowski source_model.jl:266-282, Schwarzschild :286-306, Kerr :86-103.
+template <class R> R pad_max_dt(const Scene& s, const R* ray, double max_dt_scale) {
+  const double tenth_f = (double)0.1f, half_f = 0.5, tiny_f = (double)1e-12f;
+  R d(NO_DIV_ZERO), pi(PI);
+  R eps;
+  bool kerr = s.kind >= KERR_DISK;
+  double mds = (kerr && (s.flags & REF_F32)) ? (double)(float)max_dt_scale : max_dt_scale;
+  if (ray[2] < R(PI / 8) || ray[2] > R(7 * PI / 8)) {
+    eps = R(std::min(mds, tenth_f));
+  } else if (s.kind == SCHWARZSCHILD_DISK && ray[1] <= R(1.5 * (3 * s.mass))) {
+    eps = R(max_dt_scale / 10);  // slow down near the photon sphere (:295-296)
+  } else {
+    eps = R(mds);
+  }
+  R A = kerr ? fabs(R(1) / (eps / fabs(ray[5])) + d) : fabs(R(1) / (eps / (fabs(ray[5]) + d)) + d);
+  R B = fabs(R(1) / (eps * rmin(ray[2], pi - ray[2]) / (fabs(ray[6]) + d) + d));
+  R C = fabs(R(1) / (eps / (fabs(ray[7]) + d) + d));
+  R m = rmin(R(1) / (A + B + C + d), R(half_f));
+  m = rmax(m, R(tiny_f));
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------
+// calc_planck (Minkowski source_model.jl:197-200; identical in every scene)
+template <class R> R planck(R T, R f) { return R(EMISSION_SCALE) * cube(f) / (exp(R(F_FACTOR) * f / T) - R(1)); }
+
+// Gaunt table: 146 rows (log u) x 81 columns (log gam2), Float32, loaded as
+// emission_functions.jl:15 does (data lines 44-189 of gauntff_merged_Z01.dat).
+struct GauntTable {
+  float v[146][81];
+  bool loaded = false;
+};
+inline GauntTable& gaunt_table() { static GauntTable t; return t; }
+
+// gaunt_ei_therm_van_Hoof (emission_functions.jl:54-78), Float32 as written.
+// Q7: frac is used as a 1-based Julia index, so C row = floor_i - 1.
+// Q8: ceil(frac + 1f-3) can exceed floor + 1, making the weights sum to 2.
+inline float gaunt_van_hoof(double gam2, double u) {
+  const GauntTable& t = gaunt_table();
+  float loggam = (float)std::log10(gam2);
+  float logu = (float)std::log10(u);
+  float frac_i = (logu + 16.0f) / 0.2f;
+  float frac_j = (loggam + 6.0f) / 0.2f;
+  int32_t floor_i = (int32_t)std::floor(frac_i);
+  int32_t ceil_i = (int32_t)std::ceil(frac_i + 1e-3f);
+  int32_t floor_j = (int32_t)std::floor(frac_j);
+  int32_t ceil_j = (int32_t)std::ceil(frac_j + 1e-3f);
+  if ((1 <= floor_i) && (1 <= floor_j) && (ceil_i <= 146) && (ceil_j <= 81)) {
+    auto L = [&](int i, int j) { return t.v[i - 1][j - 1]; };
+    float g1 = ((float)ceil_i - frac_i) * L(floor_i, floor_j) + (frac_i - (float)floor_i) * L(ceil_i, floor_j);
+    float g2 = ((float)ceil_i - frac_i) * L(floor_i, ceil_j) + (frac_i - (float)floor_i) * L(ceil_i, ceil_j);
+    return ((float)ceil_j - frac_j) * g1 + (frac_j - (float)floor_j) * g2;
+  }
+  return 1.2f;
+}
+
+// Matter region test (is_fire).  Minkowski source_model.jl:253-260 (r <= 1),
+// Schwarzschild :273-280 and Kerr :73-80 (wedges); the thick disk (new) is a
+// radial band with |cos theta| <= z_cut * h_over_r.
+template <class R> bool is_fire(const Scene& s, const R* x) {
+  switch (s.kind) {
+    case MINKOWSKI_SPHERE: return x[1] <= R(s.r_out);
+    case SCHWARZSCHILD_DISK:
+    case KERR_DISK:
+      return R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && R(s.th_lo) <= x[2] && x[2] <= R(s.th_hi);
+    case KERR_THICK_FF:
+      return R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && fabs(cos(x[2])) <= R(s.z_cut * s.h_over_r);
+    default: return false;
+  }
+}
+
+// get_source_velocity!: Minkowski source_model.jl:288-294, Schwarzschild :312-333,
+// Kerr :109-137 (prograde Keplerian in matter, FIDO outside).
+template <class R> void source_velocity(const Scene& s, const R* x, const R g[4][4], R u[4]) {
+  if (s.kind == MINKOWSKI_SPHERE) {
+    u[0] = R(1); u[1] = R(0); u[2] = R(0); u[3] = R(0);
+    return;
+  }
+  R M(s.mass);
+  if (s.kind == SCHWARZSCHILD_DISK) {
+    if (is_fire(s, x)) {
+      u[0] = R(1) / sqrt(R(1) - R(3) * M / x[1]);
+      u[1] = R(0); u[2] = R(0);
+      u[3] = R(1) / sqrt(R(1) - R(3) * M / x[1]) * sqrt(M / cube(x[1]));
+    } else {
+      u[0] = R(1) / sqrt(R(1) - R(2) * M / x[1]);
+      u[1] = R(0); u[2] = R(0); u[3] = R(0);
+    }
+    return;
+  }
+  R a(s.spin);
+  bool keplerian = (s.kind == KERR_SYNCH) ? false : is_fire(s, x);
+  if (keplerian) {
+    R Om = sqrt(M) / (pow(x[1], R(1.5)) + a * sqrt(M));
+    R D = pow(x[1], R(1.5)) - R(3) * M * sqrt(x[1]) + R(2) * a * sqrt(M);
+    R ut = (pow(x[1], R(1.5)) + a * sqrt(M)) / (pow(x[1], R(0.75)) * sqrt(D));
+    u[0] = ut; u[1] = R(0); u[2] = R(0); u[3] = ut * Om;
+  } else {
+    R A = sqrt(sq(x[1]) + sq(a * cos(x[2])));
+    R B = sq(x[1]) + sq(a) - R(2) * M * x[1];
+    R C = sq(sq(x[1]) + sq(a)) - B * sq(a * sin(x[2]));
+    R ut = sqrt(C) / (A * sqrt(B));
+    R Om = R(2) * a * M * x[1] / C;
+    u[0] = ut; u[1] = R(0); u[2] = R(0); u[3] = ut * Om;
+  }
+}
+
+// calc_freq_shift (gen_geometry.jl:373-390): -u_j g_ij k^i, j outer, i inner.
+template <class R> R freq_shift(const R* ray, const R u[4], const R g[4][4]) {
+  R fs(0.0);
+  for (int j = 0; j < 4; ++j) {
+    R ps(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][j] * ray[i + 4];
+    fs -= u[j] * ps;
+  }
+  return fs;
+}
+
+// Per-point fluid parameters (calc_fluid_params!, Minkowski source_model.jl:222-226:
+// T only).  For the thick disk fluid[1] = rho^2 at the point; for synchrotron the
+// scene's own set.
+template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], const R g[4][4], R fs, R* fp) {
+  fp[0] = R(s.temperature);
+  if (s.kind == KERR_THICK_FF) {
+    if (is_fire(s, x)) {
+      R c = cos(x[2]);
+      R rho = R(s.density) * exp(-(c * c) / R(2 * s.h_over_r * s.h_over_r));
+      fp[1] = rho * rho;
+    } else {
+      fp[1] = R(0);
+    }
+  }
+}
+
+// Thick-disk free-free constants: reference temperature and frequency of the
+// opacity normalisation.
+constexpr double FF_T0 = 1e4;
+constexpr double FF_NU0 = 1.0 / 550.0;
+
+// calc_radiative_coefficients (Minkowski source_model.jl:237-247, Kerr :57-67) for the
+// blackbody scenes; free-free with Kirchhoff's law for the thick disk.
+template <class R> void radiative_coefficients(const Scene& s, const R* x, const R* fp, R nu, R& a_nu, R& j_nu) {
+  if (s.kind == KERR_THICK_FF) {
+    if (is_fire(s, x)) {
+      R T = fp[0];
+      R u = R(F_FACTOR) * nu / T;
+      float gff = gaunt_van_hoof(GAM2_FACTOR / val(T), val(u));
+      if (std::is_same<R, Cnt>::value) { op_count().arith += 14; op_count().trans += 2; }
+      a_nu = R(s.absorb) * fp[1] * sqrt(R(FF_T0) / T) * cube(R(FF_NU0) / nu) * (R(1) - exp(-u)) * R((double)gff);
+      j_nu = a_nu * planck(T, nu);
+    } else {
+      a_nu = R(0); j_nu = R(0);
+    }
+    return;
+  }
+  if (is_fire(s, x)) {
+    a_nu = R(s.absorb);
+    j_nu = a_nu * planck(fp[0], nu);
+  } else {
+    a_nu = R(0); j_nu = R(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Integrator
+// ---------------------------------------------------------------------------
+struct Opts {
+  double rtol = 1e-8, atol = 1e-10;
+  double max_dt_scale = 0.1;
+  double max_steps = 1e4;
+};
+
+// Termination classes (defined by this build; the reference stops without saying why).
+enum RayClass { CLS_RUNNING = 0, CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
+
+// The ray system: state [t r th ph kt kr kth kph (I/nu^3, tau) x N] (LttM.jl:124).
+template <class R> struct RaySys {
+  const Scene* sc;
+  Opts o;
+  int ncol;
+  const double* freq;  // colors_freq = 1/lambda (nm^-1), LttM.jl:127-128
+  int n;               // raylength = 8 + 2 ncol
+  R G[4][4][4];
+  R g[4][4];
+  R u[4];
+  R fp[8];
+  RaySys(const Scene* s, const Opts& op, int nc, const double* f) : sc(s), o(op), ncol(nc), freq(f), n(8 + 2 * nc) {
+    for (int i = 0; i < 4; ++i) {
+      for (int j = 0; j < 4; ++j) {
+        g[i][j] = R(i == j ? 1.0 : 0.0);  // Matrix{Float64}(I,4,4), LttM.jl:20
+        for (int k = 0; k < 4; ++k) G[i][j][k] = R(0.0);  // zeros(4,4,4), LttM.jl:21
+      }
+      u[i] = R(0.0);
+    }
+    for (int i = 0; i < 8; ++i) fp[i] = R(0.0);
+  }
+  double atol(int) const { return o.atol; }
+  double rtol(int) const { return o.rtol; }
+
+  // calc_ray_derivative! (integrators_LttM.jl:9-48)
+  void deriv(const R* Ray, R* slope) {
+    for (int i = 0; i < 4; ++i) slope[i] = Ray[i + 4];
+    christoffel(*sc, Ray, G);
+    for (int i = 4; i < 8; ++i) {
+      R a(0.0);
+      for (int j = 4; j < 8; ++j)
+        for (int k = 4; k < 8; ++k) a -= G[i - 4][j - 4][k - 4] * Ray[j] * Ray[k];
+      slope[i] = a;
+    }
+    lower_metric(*sc, Ray, g);
+    source_velocity(*sc, Ray, g, u);
+    R fs = freq_shift(Ray, u, g);
+    fluid_params(*sc, Ray, u, g, fs, fp);
+    for (int c = 0; c < ncol; ++c) {
+      int i = 8 + 2 * c;
+      R nu = R(freq[c]) * fs;
+      R a_nu, j_nu;
+      radiative_coefficients(*sc, Ray, fp, nu, a_nu, j_nu);
+      slope[i] = -(R(1) / fs) * j_nu * exp(-Ray[i + 1]) / cube(nu);
+      slope[i + 1] = -(R(1) / fs) * a_nu;
+    }
+  }
+  R pad(const R* ray) const { return pad_max_dt(*sc, ray, o.max_dt_scale); }
+  std::pair<bool, R> near_sing(const R* ray, R h) const { return near_singularity(*sc, ray, h, o.atol, o.atol); }
+  void keep(R* ray) const { keepinbounds(*sc, ray); }
+  // calc_terminate: Minkowski source_model.jl:308-322, Schwarzschild :356-370, Kerr :168-182.
+  bool terminate(const R* ray, double max_steps, double step_count) const {
+    bool horizon = (sc->kind != MINKOWSKI_SPHERE) && (ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev));
+    if (step_count >= max_steps || ray[1] > R(sc->r_escape) || horizon || ray[3] > R(8 * PI)) return true;
+    R thr(-std::log(o.atol));
+    for (int i = 9; i < n; i += 2)
+      if (ray[i] < thr) return false;
+    return true;
+  }
+  int classify(const R* ray) const {
+    if (!isfinite(ray[1])) return CLS_BOUNDARY;
+    if (sc->kind != MINKOWSKI_SPHERE && ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev)) return CLS_HORIZON;
+    if (ray[1] > R(sc->r_escape)) return CLS_ESCAPE;
+    R thr(-std::log(o.atol));
+    bool thick = true;
+    for (int i = 9; i < n; i += 2)
+      if (!(ray[i] >= thr)) thick = false;
+    if (thick && ncol > 0) return CLS_OPAQUE;
+    return CLS_BOUNDARY;
+  }
+};
+
+// Work vectors for one ray (LttM.jl:17-36).
+template <class R> struct Work {
+  std::vector<R> y, last, next, buf, k2, k3, k4, k5, k6, k7;
+  explicit Work(int n) : y(n), last(n), next(n), buf(n), k2(n), k3(n), k4(n), k5(n), k6(n), k7(n) {}
+};
+
+// RKDP_Step_w_buffer! (integrators_LttM.jl:54-224).  Returns (new stepsize, rejected).
+template <class R, class Sys>
+std::pair<R, bool> rkdp_step(Sys& sys, R* Ray, Work<R>& w, R stepsize, bool prev_rejected, double* err_out = nullptr) {
+  const auto& B = dp().a;
+  const int n = sys.n;
+  R* k1 = w.last.data();
+  R max_dt = sys.pad(Ray);
+
+  auto ns = sys.near_sing(Ray, stepsize);
+  if (ns.first) {  // Q11: Euler fallback (:62-105)
+    R ts = ns.second;
+    for (int i = 0; i < n; ++i) Ray[i] -= ts * k1[i];
+    sys.keep(Ray);
+    for (int it = 0; it < 36; ++it) {
+      if (!sys.terminate(Ray, 2, 1)) {
+        sys.deriv(Ray, k1);
+        for (int j = 0; j < n; ++j) Ray[j] -= ts * k1[j];
+        sys.keep(Ray);
+      }
+    }
+    if (sys.terminate(Ray, 2, 1)) return {stepsize, false};
+    sys.deriv(Ray, k1);
+    stepsize = -sys.pad(Ray);
+  }
+  R* buf = w.buf.data();
+  R *k2 = w.k2.data(), *k3 = w.k3.data(), *k4 = w.k4.data(), *k5 = w.k5.data(), *k6 = w.k6.data(), *k7 = w.k7.data();
+  R* y = w.y.data();
+  R* next = w.next.data();
+  for (int i = 0; i < n; ++i) buf[i] = Ray[i] + stepsize * k1[i] * R(B[2][2]);
+  sys.deriv(buf, k2);
+  for (int i = 0; i < n; ++i) buf[i] = Ray[i] + stepsize * (k1[i] * R(B[3][2]) + k2[i] * R(B[3][3]));
+  sys.deriv(buf, k3);
+  for (int i = 0; i < n; ++i)
+    buf[i] = Ray[i] + stepsize * (k1[i] * R(B[4][2]) + k2[i] * R(B[4][3]) + k3[i] * R(B[4][4]));
+  sys.deriv(buf, k4);
+  for (int i = 0; i < n; ++i)
+    buf[i] = Ray[i] + stepsize * (k1[i] * R(B[5][2]) + k2[i] * R(B[5][3]) + k3[i] * R(B[5][4]) + k4[i] * R(B[5][5]));
+  sys.deriv(buf, k5);
+  for (int i = 0; i < n; ++i)
+    buf[i] = Ray[i] + stepsize * (k1[i] * R(B[6][2]) + k2[i] * R(B[6][3]) + k3[i] * R(B[6][4]) + k4[i] * R(B[6][5]) +
+                                  k5[i] * R(B[6][6]));
+  sys.deriv(buf, k6);
+  for (int i = 0; i < n; ++i)
+    next[i] = k1[i] * R(B[7][2]) + k3[i] * R(B[7][4]) + k4[i] * R(B[7][5]) + k5[i] * R(B[7][6]) + k6[i] * R(B[7][7]);
+  for (int i = 0; i < n; ++i) buf[i] = Ray[i] + stepsize * next[i];
+  sys.deriv(buf, k7);
+  for (int i = 0; i < n; ++i) y[i] = Ray[i] + stepsize * next[i];
+  for (int i = 0; i < n; ++i)
+    buf[i] = y[i] - Ray[i] -
+             stepsize * (k1[i] * R(B[9][2]) + k3[i] * R(B[9][4]) + k4[i] * R(B[9][5]) + k5[i] * R(B[9][6]) +
+                         k6[i] * R(B[9][7]) + k7[i] * R(B[9][8]));
+  R error(0.0);
+  for (int i = 0; i < n; ++i) {
+    R tol = R(sys.atol(i)) + rmax(fabs(Ray[i]), fabs(y[i])) * R(sys.rtol(i));
+    error += sq(buf[i] / tol);
+  }
+  error = sqrt(error / R((double)n));
+  if (err_out) *err_out = val(error);  // test hook
+  R new_stepsize = R(0.9) * stepsize * pow(R(1) / error, R(1.0 / 5.0));
+  if (fabs(new_stepsize) > max_dt) new_stepsize = -max_dt;
+  if (val(error) == 0.0 || !isfinite(new_stepsize)) {
+    sys.keep(y);
+    for (int i = 0; i < n; ++i) { k1[i] = next[i]; Ray[i] = y[i]; }
+    return {stepsize, false};
+  } else if (error > R(1)) {
+    if (prev_rejected && fabs(new_stepsize) >= fabs(stepsize)) new_stepsize = stepsize / R(2);
+    sys.keep(Ray);
+    return {new_stepsize, true};
+  } else {
+    if (fabs(new_stepsize) > R(5) * fabs(stepsize)) new_stepsize = R(5) * stepsize;
+    sys.keep(y);
+    for (int i = 0; i < n; ++i) { k1[i] = next[i]; Ray[i] = y[i]; }
+    return {new_stepsize, false};
+  }
+}
+
+struct RayStats { int64_t attempted = 0, accepted = 0; int cls = 0; };
+
+// integrate_ray (LttM.jl:11-59) without the colour step (done by calc_xyz below).
+template <class R> RayStats integrate_ray(RaySys<R>& sys, R* ray) {
+  Work<R> w(sys.n);
+  R dt = -sys.pad(ray);
+  sys.deriv(ray, w.last.data());
+  for (int i = 0; i < sys.n; ++i) w.next[i] = w.last[i];
+  bool rejected = false;
+  RayStats st;
+  double step_count = 0;
+  while (true) {
+    auto r = rkdp_step(sys, ray, w, dt, rejected);
+    dt = r.first;
+    rejected = r.second;
+    step_count += 1;
+    st.attempted++;
+    if (!rejected) st.accepted++;
+    if (sys.terminate(ray, sys.o.max_steps, step_count)) break;
+  }
+  st.cls = sys.classify(ray);
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// Colour (main/color_theory.jl, main/my_special_functions.jl:9-15)
+// ---------------------------------------------------------------------------
+inline double heaviside(double x) { return x >= 0 ? 1.0 : 0.0; }
+inline double selector(double x, double y, double z) { return y * (1 - heaviside(x)) + z * heaviside(x); }
+static const double CIE_M[4][7] = {{0.362, 1.056, -0.065, 0.821, 0.286, 1.217, 0.681},
+                                   {442.0, 599.8, 501.1, 568.8, 530.9, 437.0, 459.0},
+                                   {0.0624, 0.0264, 0.0490, 0.0213, 0.0613, 0.0845, 0.0385},
+                                   {0.0374, 0.0323, 0.0382, 0.0247, 0.0322, 0.0278, 0.0725}};
+inline double cie_lobe(double wl, int i) {
+  double d = wl - CIE_M[1][i];
+  return CIE_M[0][i] * std::exp((-1.0 / 2) * sq(d * selector(d, CIE_M[2][i], CIE_M[3][i])));
+}
+inline double cie_x(double wl) { double x = 0; for (int i = 0; i < 3; ++i) x = x + cie_lobe(wl, i); return x; }
+inline double cie_y(double wl) { double y = 0; for (int i = 0; i < 2; ++i) y = y + cie_lobe(wl, i + 3); return y; }
+inline double cie_z(double wl) { double z = 0; for (int i = 0; i < 2; ++i) z = z + cie_lobe(wl, i + 5); return z; }
+
+// precompute_CIE_weights (color_theory.jl:79-91): trapezoid weights x fits.
+inline void cie_weights(const double* lam, int n, double* WX, double* WY, double* WZ) {
+  double step = (n > 1) ? lam[1] - lam[0] : 1.0;
+  for (int i = 0; i < n; ++i) {
+    double w = 1.0 * step;
+    if (i == 0) w /= 2;
+    if (i == n - 1) w /= 2;
+    WX[i] = cie_x(lam[i]) * w;
+    WY[i] = cie_y(lam[i]) * w;
+    WZ[i] = cie_z(lam[i]) * w;
+  }
+}
+
+// ray_to_I_lambda + the X,Y,Z sums of calc_xyY (color_theory.jl:67-76, 96-102).
+inline void calc_xyz(const double* ray, const double* lam, const double* freq, int n, const double* WX,
+                     const double* WY, const double* WZ, double xyz[3]) {
+  double X = 0, Y = 0, Z = 0;
+  for (int i = 0; i < n; ++i) {
+    double I = ray[8 + 2 * i] * cube(freq[i]) * C_LIGHT / sq(lam[i]);
+    X = X + I * WX[i];
+    Y = Y + I * WY[i];
+    Z = Z + I * WZ[i];
+  }
+  xyz[0] = X; xyz[1] = Y; xyz[2] = Z;
+}
+
+// calc_xyY (color_theory.jl:103-107): black xyY(1,1,0) if any of X, Y, Z is zero.
+inline void xyz_to_xyY(const double xyz[3], double out[3]) {
+  double X = xyz[0], Y = xyz[1], Z = xyz[2];
+  if (X != 0 && Y != 0 && Z != 0) {
+    double d = X + Y + Z;
+    out[0] = X / d; out[1] = Y / d; out[2] = Y;
+  } else {
+    out[0] = 1; out[1] = 1; out[2] = 0;
+  }
+}
+
+// xyY -> XYZ -> linear sRGB (IEC 61966-2-1, D65) -> companding -> clamp -> 16 bit.
+// This stands in for Colors.jl's convert(RGB{N0f16}, xyY) (LttM.jl:203), a third-party
+// routine whose last-bit behaviour is not pinned here.
+inline double srgb_compand(double v) { return v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1 / 2.4) - 0.055; }
+inline void xyY_to_srgb16(const double xyY[3], uint16_t rgb[3]) {
+  double x = xyY[0], y = xyY[1], Yv = xyY[2];
+  double X = (y == 0) ? 0 : x * Yv / y, Z = (y == 0) ? 0 : (1 - x - y) * Yv / y;
+  double r = 3.2404542 * X - 1.5371385 * Yv - 0.4985314 * Z;
+  double g = -0.9692660 * X + 1.8760108 * Yv + 0.0415560 * Z;
+  double b = 0.0556434 * X - 0.2040259 * Yv + 1.0572252 * Z;
+  double c[3] = {r, g, b};
+  for (int i = 0; i < 3; ++i) {
+    double v = std::min(1.0, std::max(0.0, srgb_compand(c[i])));
+    rgb[i] = (uint16_t)std::lround(v * 65535.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Camera (main/gen_geometry.jl:9-103, main/initialization.jl:190-331)
+// ---------------------------------------------------------------------------
+typedef double M4[4][4];
+inline void m4_mul(const M4 A, const M4 B, M4 C) {
+  M4 T;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double s = 0;
+      for (int k = 0; k < 4; ++k) s += A[i][k] * B[k][j];
+      T[i][j] = s;
+    }
+  std::memcpy(C, T, sizeof(M4));
+}
+inline void m4_zero(M4 A) { std::memset(A, 0, sizeof(M4)); }
+inline void m4_eye(M4 A) { m4_zero(A); for (int i = 0; i < 4; ++i) A[i][i] = 1; }
+
+// gen_intrinsic_rotation_matrix (gen_geometry.jl:9-40): R = R_z R_y R_x.
+inline void intrinsic_rotation(const double p[3], M4 R) {
+  M4 Rx, Ry, Rz;
+  m4_zero(Rx); m4_zero(Ry); m4_zero(Rz);
+  Rx[0][0] = 1; Rx[1][1] = 1; Rx[2][2] = std::cos(p[0]); Rx[3][2] = std::sin(p[0]);
+  Rx[2][3] = -std::sin(p[0]); Rx[3][3] = std::cos(p[0]);
+  Ry[0][0] = 1; Ry[1][1] = std::cos(p[1]); Ry[3][1] = -std::sin(p[1]); Ry[2][2] = 1;
+  Ry[1][3] = std::sin(p[1]); Ry[3][3] = std::cos(p[1]);
+  Rz[0][0] = 1; Rz[1][1] = std::cos(p[2]); Rz[2][1] = std::sin(p[2]); Rz[1][2] = -std::sin(p[2]);
+  Rz[2][2] = std::cos(p[2]); Rz[3][3] = 1;
+  M4 T;
+  m4_mul(Rz, Ry, T);
+  m4_mul(T, Rx, R);
+}
+
+// gen_final_rotation_matrix (gen_geometry.jl:47-87).
+inline void final_rotation(const double dir[2], M4 R) {
+  double d[3] = {std::sin(dir[0]) * std::cos(dir[1]), std::sin(dir[0]) * std::sin(dir[1]), std::cos(dir[0])};
+  double k[3] = {0 * d[2] - 1 * d[1], 1 * d[0] - 0 * d[2], 0 * d[1] - 0 * d[0]};  // cross([0,0,1], d)
+  double mag = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  double p[3] = {0, 0, PI / 2 + dir[1]};
+  M4 Rz;
+  intrinsic_rotation(p, Rz);
+  if (mag > 0) {
+    for (int i = 0; i < 3; ++i) k[i] /= mag;
+    double K[3][3] = {{0, -k[2], k[1]}, {k[2], 0, -k[0]}, {-k[1], k[0], 0}};
+    double KK[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0;
+        for (int m = 0; m < 3; ++m) s += ((1 - std::cos(dir[0])) * K[i][m]) * K[m][j];
+        KK[i][j] = s;
+      }
+    M4 R4;
+    m4_zero(R4);
+    R4[0][0] = 1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) R4[1 + j][1 + i] = (i == j ? 1.0 : 0.0) + std::sin(dir[0]) * K[i][j] + KK[i][j];  // transposed
+    m4_mul(R4, Rz, R);
+  } else {
+    std::memcpy(R, Rz, sizeof(M4));
+  }
+}
+
+// lorentz_boost_z (gen_geometry.jl:93-103).
+inline void boost_z(double beta, M4 L) {
+  m4_zero(L);
+  double gamma = 1 / std::sqrt(1 - beta * beta);
+  L[0][0] = gamma; L[3][0] = -beta * gamma; L[0][3] = -beta * gamma;
+  L[1][1] = 1; L[2][2] = 1; L[3][3] = gamma;
+}
+
+// calc_inv_vierbein: Minkowski source_geometry.jl:45-50, Schwarzschild :53-60, Kerr :112-121.
+inline void inv_vierbein(const Scene& s, const double* x, M4 e) {
+  m4_eye(e);
+  if (s.kind == MINKOWSKI_SPHERE) {
+    e[2][2] = 1 / (x[1] + NO_DIV_ZERO);
+    e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
+  } else if (s.kind == SCHWARZSCHILD_DISK) {
+    e[0][0] = 1 / std::sqrt(1.0 - s.r_s / x[1]);
+    e[1][1] = std::sqrt(1.0 - s.r_s / x[1]);
+    e[2][2] = 1 / x[1];
+    e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
+  } else {
+    double a = s.spin, rs = s.r_s;
+    double Sig = sq(x[1]) + sq(a * std::cos(x[2]));
+    double Del = sq(x[1]) - x[1] * rs + sq(a);
+    double A = (sq(x[1]) + sq(a)) * Sig + rs * x[1] * sq(a * std::sin(x[2]));
+    e[0][0] = std::sqrt(A / (Sig * Del));
+    e[3][0] = (rs * a * x[1] / A) * e[0][0];
+    e[1][1] = std::sqrt(Del / Sig);
+    e[2][2] = 1 / std::sqrt(Sig);
+    e[3][3] = std::sqrt(Sig / A) / std::sin(x[2]);
+  }
+}
+
+struct Camera {
+  double pos[4] = {0, 0, 0, 0};
+  double dir[2] = {0, 0};
+  double beta = 0;
+  double point[3] = {0, 0, 0};
+  int x_pix = 100;
+  double fov_hor = 85, fov_vert = 60;  // degrees
+};
+
+// y_pix = round(Int, x_pix * H / W) (initialization.jl:200), Julia rounds ties to even.
+inline int camera_y_pix(const Camera& c) {
+  double fh = PI * c.fov_hor / 180, fv = PI * c.fov_vert / 180;
+  double W = std::tan(fh / 2), H = std::tan(fv / 2);
+  return (int)std::nearbyint(c.x_pix * H / W);
+}
+
+// The per-render transform e * R2 * L * R (initialization.jl:253-260, 327).
+inline void camera_matrix(const Scene& s, const Camera& c, M4 Mout) {
+  M4 R, L, R2, e, T;
+  intrinsic_rotation(c.point, R);
+  boost_z(-c.beta, L);
+  final_rotation(c.dir, R2);
+  inv_vierbein(s, c.pos, e);
+  m4_mul(e, R2, T);
+  m4_mul(T, L, T);
+  m4_mul(T, R, Mout);
+}
+
+// initialize_camera (initialization.jl:190-331) for pixel (i, j) 0-based, ray index
+// i + j * x_pix (column-major (x_pix, y_pix), LttM.jl:141).  Writes the first 8
+// components; the rest are zero.
+inline void camera_ray(const Camera& c, const M4 Mc, int i, int j, int y_pix, double* ray8) {
+  double fh = PI * c.fov_hor / 180, fv = PI * c.fov_vert / 180;
+  double W = std::tan(fh / 2), H = std::tan(fv / 2);
+  double rh = 2 * W / (c.x_pix + 1), rv = 2 * H / (y_pix + 1);
+  double w = rh * ((i + 1) - 0.5) - W;
+  double h = rv * ((j + 1) - 0.5) - H;
+  double th = std::atan(std::sqrt(w * w + h * h));
+  double ph;
+  if (w > 0) ph = std::atan(h / w);
+  else if (w < 0 && h >= 0) ph = std::atan(h / w) + PI;
+  else if (w < 0 && h < 0) ph = std::atan(h / w) - PI;
+  else if (w == 0 && h > 0) ph = PI / 2;
+  else if (w == 0 && h < 0) ph = -PI / 2;
+  else ph = 0;
+  double v[4] = {1, -std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+  for (int k = 0; k < 4; ++k) ray8[k] = c.pos[k];
+  for (int a = 0; a < 4; ++a) {
+    double sacc = 0;
+    for (int b = 0; b < 4; ++b) sacc += Mc[a][b] * v[b];
+    ray8[4 + a] = sacc;
+  }
+}
+
+// is_singularity (Kerr source_geometry.jl:307-315 etc.): camera validity guard.
+inline bool is_singularity(const double* x) {
+  return std::fabs(x[1]) <= (double)1e-32f || std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - PI) <= 1e-32;
+}
+
+}  // namespace rtwo

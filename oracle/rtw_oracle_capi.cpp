@@ -1,0 +1,326 @@
+// rtw_oracle_capi.cpp -- extern "C" wrapper of the oracle (rtw_oracle.hpp) for ctypes.
+//
+// TEST INFRASTRUCTURE ONLY (see rtw_oracle.hpp).  Built into oracle/liboracle.so by
+// oracle/Makefile with -ffp-contract=off.  The parameter structs are the public ones
+// of include/rtw.h so the oracle and librtw read identical inputs.
+#include "rtw_oracle.hpp"
+#include "../include/rtw.h"
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <thread>
+
+using namespace rtwo;
+
+static Scene to_scene(const rtw_scene* s) {
+  Scene o;
+  o.kind = s->kind; o.flags = s->flags; o.mass = s->mass; o.spin = s->spin;
+  o.temperature = s->temperature; o.absorb = s->absorb; o.density = s->density;
+  o.r_in = s->r_in; o.r_out = s->r_out; o.th_lo = s->th_lo; o.th_hi = s->th_hi;
+  o.h_over_r = s->h_over_r; o.z_cut = s->z_cut; o.r_escape = s->r_escape;
+  o.bfield = s->bfield; o.theta_e = s->theta_e; o.jet_half_angle = s->jet_half_angle;
+  o.r_jet = s->r_jet; o.torus_r_max = s->torus_r_max;
+  scene_init(o);
+  return o;
+}
+static Camera to_cam(const rtw_camera* c) {
+  Camera o;
+  for (int i = 0; i < 4; ++i) o.pos[i] = c->pos[i];
+  o.dir[0] = c->dir[0]; o.dir[1] = c->dir[1]; o.beta = c->beta;
+  for (int i = 0; i < 3; ++i) o.point[i] = c->point[i];
+  o.x_pix = c->x_pix; o.fov_hor = c->fov_hor; o.fov_vert = c->fov_vert;
+  return o;
+}
+static Opts to_opts(const rtw_integrator* p) {
+  Opts o;
+  o.rtol = p->rtol; o.atol = p->atol; o.max_dt_scale = p->max_dt_scale; o.max_steps = p->max_steps;
+  return o;
+}
+
+extern "C" {
+
+// Parse the van Hoof table the way emission_functions.jl:15 does: skip 42 lines, then
+// skip comment lines, then 146 rows of 81 numbers, stored as Float32.
+int rtwo_load_gaunt(const char* path) {
+  std::ifstream f(path);
+  if (!f) return -1;
+  std::string line;
+  for (int i = 0; i < 42 && std::getline(f, line); ++i) {}
+  int row = 0;
+  GauntTable& t = gaunt_table();
+  while (row < 146 && std::getline(f, line)) {
+    size_t p = line.find_first_not_of(" \t");
+    if (p == std::string::npos || line[p] == '#') continue;
+    std::istringstream is(line);
+    for (int j = 0; j < 81; ++j) {
+      double v;
+      if (!(is >> v)) return -2;
+      t.v[row][j] = (float)v;
+    }
+    ++row;
+  }
+  if (row != 146) return -3;
+  t.loaded = true;
+  return 0;
+}
+
+float rtwo_gaunt(double gam2, double u) { return gaunt_van_hoof(gam2, u); }
+double rtwo_planck(double T, double f) { return planck<double>(T, f); }
+double rtwo_isco_ref(double a, double M, int f32) { return kerr_isco_ref(a, M, f32 != 0); }
+double rtwo_circ_beta(double r, double a, double M) { return kerr_circ_beta(r, a, M); }
+double rtwo_gam2_factor(void) { return GAM2_FACTOR; }
+
+void rtwo_scene_derived(const rtw_scene* s, double* out /*r_s, r_ev, spin*/) {
+  Scene sc = to_scene(s);
+  out[0] = sc.r_s; out[1] = sc.r_ev; out[2] = sc.spin;
+}
+
+void rtwo_metric(const rtw_scene* s, const double* x, double* out16) {
+  Scene sc = to_scene(s);
+  double g[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) g[i][j] = (i == j);
+  lower_metric<double>(sc, x, g);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out16[i * 4 + j] = g[i][j];
+}
+
+void rtwo_christoffel(const rtw_scene* s, const double* x, double* out64) {
+  Scene sc = to_scene(s);
+  double G[4][4][4];
+  std::memset(G, 0, sizeof(G));
+  christoffel<double>(sc, x, G);
+  std::memcpy(out64, G, sizeof(G));
+}
+
+double rtwo_pad_max_dt(const rtw_scene* s, const double* ray8, double mds) {
+  Scene sc = to_scene(s);
+  return pad_max_dt<double>(sc, ray8, mds);
+}
+
+int rtwo_near_singularity(const rtw_scene* s, const double* ray8, double h, double atol, double* out_h) {
+  Scene sc = to_scene(s);
+  auto r = near_singularity<double>(sc, ray8, h, atol, atol);
+  *out_h = r.second;
+  return r.first ? 1 : 0;
+}
+
+void rtwo_keepinbounds(const rtw_scene* s, double* ray8) {
+  Scene sc = to_scene(s);
+  keepinbounds<double>(sc, ray8);
+}
+
+void rtwo_source_velocity(const rtw_scene* s, const double* x, double* u4) {
+  Scene sc = to_scene(s);
+  double g[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) g[i][j] = (i == j);
+  lower_metric<double>(sc, x, g);
+  source_velocity<double>(sc, x, g, u4);
+}
+
+// calc_ray_derivative! at one state (n = 8 + 2 nlam components).
+void rtwo_derivative(const rtw_scene* s, const rtw_integrator* o, const double* lam, int nlam, const double* ray,
+                     double* slope) {
+  Scene sc = to_scene(s);
+  std::vector<double> freq(nlam);
+  for (int i = 0; i < nlam; ++i) freq[i] = 1.0 / lam[i];
+  RaySys<double> sys(&sc, to_opts(o), nlam, freq.data());
+  sys.deriv(ray, slope);
+}
+
+int rtwo_terminate(const rtw_scene* s, const rtw_integrator* o, int nlam, const double* ray, double step_count) {
+  Scene sc = to_scene(s);
+  RaySys<double> sys(&sc, to_opts(o), nlam, nullptr);
+  return sys.terminate(ray, o->max_steps, step_count) ? 1 : 0;
+}
+
+int rtwo_classify(const rtw_scene* s, const rtw_integrator* o, int nlam, const double* ray) {
+  Scene sc = to_scene(s);
+  RaySys<double> sys(&sc, to_opts(o), nlam, nullptr);
+  return sys.classify(ray);
+}
+
+void rtwo_dp_tableau(double* out90) {
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 8; ++j) out90[i * 8 + j] = dp().a[i + 1][j + 1];
+}
+
+void rtwo_cie_weights(const double* lam, int n, double* WX, double* WY, double* WZ) { cie_weights(lam, n, WX, WY, WZ); }
+void rtwo_cie_xyz_fit(double wl, double* out3) { out3[0] = cie_x(wl); out3[1] = cie_y(wl); out3[2] = cie_z(wl); }
+
+void rtwo_ray_xyz(const double* ray, const double* lam, int n, double* xyz) {
+  std::vector<double> f(n), WX(n), WY(n), WZ(n);
+  for (int i = 0; i < n; ++i) f[i] = 1.0 / lam[i];
+  cie_weights(lam, n, WX.data(), WY.data(), WZ.data());
+  calc_xyz(ray, lam, f.data(), n, WX.data(), WY.data(), WZ.data(), xyz);
+}
+
+void rtwo_xyz_to_xyY(const double* xyz, double* out) { xyz_to_xyY(xyz, out); }
+void rtwo_xyY_to_srgb16(const double* xyY, uint16_t* out) { xyY_to_srgb16(xyY, out); }
+
+int rtwo_camera_y_pix(const rtw_camera* c) { return camera_y_pix(to_cam(c)); }
+void rtwo_camera_matrix(const rtw_scene* s, const rtw_camera* c, double* out16) {
+  Scene sc = to_scene(s);
+  M4 M;
+  camera_matrix(sc, to_cam(c), M);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) out16[i * 4 + j] = M[i][j];
+}
+// Column-major (num_pix, 8) initial rays.
+void rtwo_camera_rays(const rtw_scene* s, const rtw_camera* c, double* out) {
+  Scene sc = to_scene(s);
+  Camera cam = to_cam(c);
+  int ny = camera_y_pix(cam);
+  int64_t np = (int64_t)cam.x_pix * ny;
+  M4 M;
+  camera_matrix(sc, cam, M);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < cam.x_pix; ++i) {
+      double r8[8];
+      camera_ray(cam, M, i, j, ny, r8);
+      int64_t p = i + (int64_t)j * cam.x_pix;
+      for (int k = 0; k < 8; ++k) out[p + k * np] = r8[k];
+    }
+}
+
+// Scalar test ODE y' = lam * y driven through rkdp_step (integrators_LttM.jl:54-224).
+struct LinSys {
+  int n = 1;
+  double lam, mdt, at, rt;
+  void deriv(const double* y, double* s) { s[0] = lam * y[0]; }
+  double pad(const double*) const { return mdt; }
+  std::pair<bool, double> near_sing(const double*, double h) const { return {false, h}; }
+  void keep(double*) const {}
+  bool terminate(const double*, double, double) const { return false; }
+  double atol(int) const { return at; }
+  double rtol(int) const { return rt; }
+};
+// Runs nsteps calls; records per call: y after, stepsize returned, rejected, last_slope after.
+void rtwo_ode_trace(double lam, double y0, double h0, double max_dt, double atol, double rtol, int nsteps,
+                    double* out_y, double* out_h, int* out_rej, double* out_k1, double* out_err) {
+  LinSys sys;
+  sys.lam = lam; sys.mdt = max_dt; sys.at = atol; sys.rt = rtol;
+  Work<double> w(1);
+  double y = y0;
+  sys.deriv(&y, w.last.data());
+  double h = h0;
+  bool rej = false;
+  for (int s = 0; s < nsteps; ++s) {
+    double e = 0;
+    auto r = rkdp_step(sys, &y, w, h, rej, &e);
+    h = r.first; rej = r.second;
+    out_y[s] = y; out_h[s] = h; out_rej[s] = rej; out_k1[s] = w.last[0];
+    if (out_err) out_err[s] = e;
+  }
+}
+
+// Render (integrate + XYZ) with nthreads threads in dynamic chunks of
+// cld(npix, 100 * nthreads) rays (LttM.jl:156-165).  subset: optional list of ray
+// indices to integrate (others untouched); init_rays: optional column-major
+// (npix, 8) initial states.  Outputs indexed by ray index.
+int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o, const double* lam, int nlam,
+                int nthreads, const int64_t* subset, int64_t nsub, const double* init_rays, double* out_xyz,
+                uint8_t* out_cls, int64_t* out_steps, double* out_rays, double* out_seconds) {
+  if (!gaunt_table().loaded && s->kind == KERR_THICK_FF) return -10;
+  Scene sc = to_scene(s);
+  Camera cam = to_cam(c);
+  Opts op = to_opts(o);
+  int ny = camera_y_pix(cam);
+  int64_t np = (int64_t)cam.x_pix * ny;
+  int n = 8 + 2 * nlam;
+  std::vector<double> freq(nlam), WX(nlam), WY(nlam), WZ(nlam);
+  for (int i = 0; i < nlam; ++i) freq[i] = 1.0 / lam[i];
+  cie_weights(lam, nlam, WX.data(), WY.data(), WZ.data());
+  M4 M;
+  camera_matrix(sc, cam, M);
+  int64_t total = subset ? nsub : np;
+  if (nthreads < 1) nthreads = 1;
+  int64_t chunk = (total + 100LL * nthreads - 1) / (100LL * nthreads);
+  if (chunk < 1) chunk = 1;
+  std::atomic<int64_t> next(0);
+  auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&]() {
+    std::vector<double> ray(n);
+    while (true) {
+      int64_t b = next.fetch_add(chunk);
+      if (b >= total) break;
+      int64_t e = std::min(total, b + chunk);
+      for (int64_t q = b; q < e; ++q) {
+        int64_t p = subset ? subset[q] : q;
+        std::fill(ray.begin(), ray.end(), 0.0);
+        if (init_rays) {
+          for (int k = 0; k < 8; ++k) ray[k] = init_rays[p + k * np];
+        } else {
+          camera_ray(cam, M, (int)(p % cam.x_pix), (int)(p / cam.x_pix), ny, ray.data());
+        }
+        RaySys<double> sys(&sc, op, nlam, freq.data());
+        RayStats st = integrate_ray(sys, ray.data());
+        double xyz[3];
+        calc_xyz(ray.data(), lam, freq.data(), nlam, WX.data(), WY.data(), WZ.data(), xyz);
+        if (out_xyz) for (int k = 0; k < 3; ++k) out_xyz[3 * p + k] = xyz[k];
+        if (out_cls) out_cls[p] = (uint8_t)st.cls;
+        if (out_steps) { out_steps[2 * p] = st.attempted; out_steps[2 * p + 1] = st.accepted; }
+        if (out_rays) for (int k = 0; k < n; ++k) out_rays[p + k * np] = ray[k];
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nthreads; ++t) th.emplace_back(worker);
+  worker();
+  for (auto& t : th) t.join();
+  auto t1 = std::chrono::steady_clock::now();
+  if (out_seconds) *out_seconds = std::chrono::duration<double>(t1 - t0).count();
+  return 0;
+}
+
+// Arithmetic work of attempted RK45 steps, measured with the counting scalar on
+// the rays in subset: returns totals (arith ops, transcendental calls, attempted steps).
+int rtwo_count_ops(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o, const double* lam, int nlam,
+                   const int64_t* subset, int64_t nsub, double* out3) {
+  Scene sc = to_scene(s);
+  Camera cam = to_cam(c);
+  Opts op = to_opts(o);
+  int ny = camera_y_pix(cam);
+  int n = 8 + 2 * nlam;
+  std::vector<double> freq(nlam);
+  for (int i = 0; i < nlam; ++i) freq[i] = 1.0 / lam[i];
+  M4 M;
+  camera_matrix(sc, cam, M);
+  uint64_t ar0 = op_count().arith, tr0 = op_count().trans;
+  int64_t steps = 0;
+  std::vector<double> r8(n);
+  std::vector<Cnt> ray(n);
+  for (int64_t q = 0; q < nsub; ++q) {
+    int64_t p = subset[q];
+    std::fill(r8.begin(), r8.end(), 0.0);
+    camera_ray(cam, M, (int)(p % cam.x_pix), (int)(p / cam.x_pix), ny, r8.data());
+    for (int k = 0; k < n; ++k) ray[k] = Cnt(r8[k]);
+    RaySys<Cnt> sys(&sc, op, nlam, freq.data());
+    uint64_t a0 = op_count().arith, t0 = op_count().trans;
+    // exclude the one-off initial derivative and pad from the per-step figure
+    Work<Cnt> w(n);
+    Cnt dt = -sys.pad(ray.data());
+    sys.deriv(ray.data(), w.last.data());
+    op_count().arith = a0; op_count().trans = t0;
+    bool rej = false;
+    double sc_ = 0;
+    while (true) {
+      auto r = rkdp_step(sys, ray.data(), w, dt, rej);
+      dt = r.first; rej = r.second;
+      sc_ += 1;
+      steps++;
+      if (sys.terminate(ray.data(), op.max_steps, sc_)) break;
+    }
+  }
+  out3[0] = (double)(op_count().arith - ar0);
+  out3[1] = (double)(op_count().trans - tr0);
+  out3[2] = (double)steps;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+"""ctypes mirror of include/rtw.h (structs and constants).
+
+The struct layouts must match rtw.h field for field; tests/test_abi.py checks
+the sizes against the C compiler's.
+"""
+import ctypes as C
+
+RTW_ABI_VERSION = 1
+
+RTW_OK = 0
+RTW_E_ARG = -1
+RTW_E_DEVICE = -2
+RTW_E_NOMEM = -3
+RTW_E_CAMERA = -4
+RTW_E_UNSUPPORTED = -5
+
+SCENE_MINKOWSKI_SPHERE = 1
+SCENE_SCHWARZSCHILD_DISK = 2
+SCENE_KERR_DISK = 3
+SCENE_KERR_THICK_FF = 4
+SCENE_KERR_SYNCH = 5
+
+SCENE_REF_F32 = 1
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+SCHED_GRID = 0
+SCHED_PERSISTENT = 1
+
+CLASS_HORIZON = 1
+CLASS_ESCAPE = 2
+CLASS_OPAQUE = 3
+CLASS_BOUNDARY = 4
+CLASS_NAMES = {1: "horizon", 2: "escape", 3: "opaque", 4: "boundary"}
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("flags", C.c_int32),
+        ("mass", C.c_double), ("spin", C.c_double), ("temperature", C.c_double),
+        ("absorb", C.c_double), ("density", C.c_double),
+        ("r_in", C.c_double), ("r_out", C.c_double),
+        ("th_lo", C.c_double), ("th_hi", C.c_double),
+        ("h_over_r", C.c_double), ("z_cut", C.c_double), ("r_escape", C.c_double),
+        ("bfield", C.c_double), ("theta_e", C.c_double), ("jet_half_angle", C.c_double),
+        ("r_jet", C.c_double), ("torus_r_max", C.c_double),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("pos", C.c_double * 4), ("dir", C.c_double * 2), ("beta", C.c_double),
+        ("point", C.c_double * 3), ("x_pix", C.c_int32), ("reserved", C.c_int32),
+        ("fov_hor", C.c_double), ("fov_vert", C.c_double),
+    ]
+
+
+class Integrator(C.Structure):
+    _fields_ = [
+        ("rtol", C.c_double), ("atol", C.c_double), ("max_dt_scale", C.c_double),
+        ("max_steps", C.c_double), ("precision", C.c_int32), ("schedule", C.c_int32),
+        ("fp64_radius", C.c_double), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_int64),
+        ("steps_attempted", C.c_int64), ("steps_accepted", C.c_int64),
+        ("steps_fp64", C.c_int64), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+def make_camera(pos, direction=(0.0, 0.0), beta=0.0, pointing=(0.0, 0.0, 0.0),
+                x_pix=100, fov_hor=85.0, fov_vert=60.0):
+    c = Camera()
+    for i in range(4):
+        c.pos[i] = float(pos[i])
+    c.dir[0], c.dir[1] = float(direction[0]), float(direction[1])
+    c.beta = float(beta)
+    for i in range(3):
+        c.point[i] = float(pointing[i])
+    c.x_pix = int(x_pix)
+    c.fov_hor = float(fov_hor)
+    c.fov_vert = float(fov_vert)
+    return c
+
+
+def make_integrator(rtol=1e-4, atol=None, max_dt_scale=0.1, max_steps=1e3,
+                    precision=PREC_FP64, schedule=SCHED_PERSISTENT, fp64_radius=3.0,
+                    lanes_per_ray=0):
+    o = Integrator()
+    o.rtol = float(rtol)
+    o.atol = float(rtol if atol is None else atol)
+    o.max_dt_scale = float(max_dt_scale)
+    o.max_steps = float(max_steps)
+    o.precision = int(precision)
+    o.schedule = int(schedule)
+    o.fp64_radius = float(fp64_radius)
+    o.lanes_per_ray = int(lanes_per_ray)
+    return o
+
+
+def scene_copy(s):
+    n = Scene()
+    C.memmove(C.byref(n), C.byref(s), C.sizeof(Scene))
+    return n

@@ -1,0 +1,579 @@
+// rtw_api.hip -- C ABI of librtw (include/rtw.h): context, parameter derivation,
+// kernel dispatch, the XYZ -> sRGB reduction.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/rtw.h"
+#include "rtw_launch.h"
+
+using namespace rtw;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct rtw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cu = 0;
+  std::string err;
+  bool gaunt_loaded = false;
+  std::vector<float> gaunt;  // 146 x 81 row-major
+  DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs;
+};
+
+static thread_local std::string g_create_err;
+
+static int fail(rtw_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+#define HIPCHK(ctx, call)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? RTW_E_NOMEM : RTW_E_DEVICE,                 \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                          \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// host-side derivation of the per-render constants
+// ---------------------------------------------------------------------------
+typedef double M4[4][4];
+static void mmul(const M4 A, const M4 B, M4 C) {
+  M4 T;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double s = 0;
+      for (int k = 0; k < 4; ++k) s += A[i][k] * B[k][j];
+      T[i][j] = s;
+    }
+  std::memcpy(C, T, sizeof(M4));
+}
+static void mzero(M4 A) { std::memset(A, 0, sizeof(M4)); }
+
+// roll/pitch/yaw: gen_intrinsic_rotation_matrix (main/gen_geometry.jl:9-40), R = Rz Ry Rx
+static void rot_intrinsic(const double p[3], M4 R) {
+  M4 X, Y, Z;
+  mzero(X); mzero(Y); mzero(Z);
+  double c0 = std::cos(p[0]), s0 = std::sin(p[0]), c1 = std::cos(p[1]), s1 = std::sin(p[1]);
+  double c2 = std::cos(p[2]), s2 = std::sin(p[2]);
+  X[0][0] = 1; X[1][1] = 1; X[2][2] = c0; X[3][2] = s0; X[2][3] = -s0; X[3][3] = c0;
+  Y[0][0] = 1; Y[1][1] = c1; Y[3][1] = -s1; Y[2][2] = 1; Y[1][3] = s1; Y[3][3] = c1;
+  Z[0][0] = 1; Z[1][1] = c2; Z[2][1] = s2; Z[1][2] = -s2; Z[2][2] = c2; Z[3][3] = 1;
+  M4 T;
+  mmul(Z, Y, T);
+  mmul(T, X, R);
+}
+
+// gen_final_rotation_matrix (main/gen_geometry.jl:47-87): the transposed
+// Rodrigues rotation taking z to the velocity direction, then a yaw of
+// pi/2 + direction[2].
+static void rot_final(const double dir[2], M4 R) {
+  double d[3] = {std::sin(dir[0]) * std::cos(dir[1]), std::sin(dir[0]) * std::sin(dir[1]), std::cos(dir[0])};
+  double k[3] = {-d[1], d[0], 0.0};
+  double mk = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+  double yaw[3] = {0, 0, M_PI / 2 + dir[1]};
+  M4 Rz;
+  rot_intrinsic(yaw, Rz);
+  if (mk > 0) {
+    for (double& v : k) v /= mk;
+    double K[3][3] = {{0, -k[2], k[1]}, {k[2], 0, -k[0]}, {-k[1], k[0], 0}};
+    double sd = std::sin(dir[0]), cd1 = 1 - std::cos(dir[0]);
+    M4 Rt;
+    mzero(Rt);
+    Rt[0][0] = 1;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double kk = 0;
+        for (int m = 0; m < 3; ++m) kk += (cd1 * K[i][m]) * K[m][j];
+        Rt[1 + j][1 + i] = (i == j ? 1.0 : 0.0) + sd * K[i][j] + kk;
+      }
+    mmul(Rt, Rz, R);
+  } else {
+    std::memcpy(R, Rz, sizeof(M4));
+  }
+}
+
+// lorentz_boost_z (main/gen_geometry.jl:93-103)
+static void boost(double b, M4 L) {
+  mzero(L);
+  double g = 1 / std::sqrt(1 - b * b);
+  L[0][0] = g; L[3][0] = -b * g; L[0][3] = -b * g; L[1][1] = 1; L[2][2] = 1; L[3][3] = g;
+}
+
+// calc_inv_vierbein (Minkowski source_geometry.jl:45-50; Schwarzschild :53-60; Kerr :112-121)
+static void inv_vierbein(const DevScene& S, const double* x, M4 e) {
+  mzero(e);
+  for (int i = 0; i < 4; ++i) e[i][i] = 1;
+  if (S.kind == MINK) {
+    e[2][2] = 1 / (x[1] + NO_DIV_ZERO);
+    e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
+  } else if (S.kind == SCHW) {
+    double f = 1.0 - S.rs / x[1];
+    e[0][0] = 1 / std::sqrt(f);
+    e[1][1] = std::sqrt(f);
+    e[2][2] = 1 / x[1];
+    e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
+  } else {
+    double a = S.a, r = x[1], s = std::sin(x[2]), c = std::cos(x[2]);
+    double Sg = r * r + (a * c) * (a * c), D = r * r - r * S.rs + a * a;
+    double A = (r * r + a * a) * Sg + S.rs * r * (a * s) * (a * s);
+    e[0][0] = std::sqrt(A / (Sg * D));
+    e[3][0] = (S.rs * a * r / A) * e[0][0];
+    e[1][1] = std::sqrt(D / Sg);
+    e[2][2] = 1 / std::sqrt(Sg);
+    e[3][3] = std::sqrt(Sg / A) / s;
+  }
+}
+
+static bool cam_is_singular(const double* x) {
+  return std::fabs(x[1]) <= (double)1e-32f || std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - M_PI) <= 1e-32;
+}
+
+static int cam_y_pix(const rtw_camera& c) {
+  double W = std::tan(M_PI * c.fov_hor / 180 / 2), H = std::tan(M_PI * c.fov_vert / 180 / 2);
+  return (int)std::nearbyint(c.x_pix * H / W);
+}
+
+static DevCam make_cam(const DevScene& S, const rtw_camera& c) {
+  DevCam d;
+  M4 R, L, R2, e, T, M;
+  rot_intrinsic(c.point, R);
+  boost(-c.beta, L);
+  rot_final(c.dir, R2);
+  inv_vierbein(S, c.pos, e);
+  mmul(e, R2, T);
+  mmul(T, L, T);
+  mmul(T, R, M);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) d.Mc[i * 4 + j] = M[i][j];
+  for (int i = 0; i < 4; ++i) d.pos[i] = c.pos[i];
+  int ny = cam_y_pix(c);
+  d.W = std::tan(M_PI * c.fov_hor / 180 / 2);
+  d.H = std::tan(M_PI * c.fov_vert / 180 / 2);
+  d.rho_h = 2 * d.W / (c.x_pix + 1);
+  d.rho_v = 2 * d.H / (ny + 1);
+  return d;
+}
+
+// Scene constants (main/Kerr.source/source_geometry.jl:7-17 and the scene model files).
+static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o, DevScene& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.kind = s.kind;
+  d.flags = s.flags;
+  d.M = s.mass;
+  d.a = s.spin;
+  d.rs = 2 * s.mass;
+  d.T = s.temperature;
+  d.absorb = s.absorb;
+  d.density = s.density;
+  d.r_in = s.r_in; d.r_out = s.r_out; d.th_lo = s.th_lo; d.th_hi = s.th_hi;
+  d.cos_cut = s.z_cut * s.h_over_r;
+  d.inv_2h2 = 1.0 / (2 * s.h_over_r * s.h_over_r);
+  d.r_escape = s.r_escape;
+  d.r_photon15 = 1.5 * (3 * s.mass);
+  d.ref_f32 = (s.kind >= RTW_SCENE_KERR_DISK && (s.flags & RTW_SCENE_REF_F32)) ? 1 : 0;
+  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_KERR_THICK_FF)
+    return fail(ctx, RTW_E_UNSUPPORTED, "scene kind not supported");
+  if (s.kind == RTW_SCENE_SCHWARZSCHILD_DISK) {
+    d.r_ev = d.rs;
+  } else if (s.kind >= RTW_SCENE_KERR_DISK) {
+    if (d.ref_f32) {
+      float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
+      d.r_ev = (double)(rs / 2 + std::sqrt(rs * rs / 4 - a * a));
+      d.a = (double)a;
+      d.rs = (double)rs;
+    } else {
+      d.r_ev = d.rs / 2 + std::sqrt(d.rs * d.rs / 4 - d.a * d.a);
+    }
+    if (!(std::fabs(d.a) < s.mass)) return fail(ctx, RTW_E_ARG, "|spin| must be < M");
+  }
+  d.r_horizon = (s.kind == RTW_SCENE_MINKOWSKI_SPHERE) ? 0.0 : (1 + 10 * o.rtol) * d.r_ev;
+  // pad_max_dt epsilons: Kerr uses Float32(max_dt_scale) (Kerr.source/source_model.jl:93-95)
+  double mds = (d.ref_f32) ? (double)(float)o.max_dt_scale : o.max_dt_scale;
+  const double tenth_f = (double)0.1f;
+  d.eps_main = mds;
+  d.eps_pole = mds < tenth_f ? mds : tenth_f;
+  d.eps_photon = o.max_dt_scale / 10;
+  d.bb_a = s.absorb;
+  if (s.kind == RTW_SCENE_KERR_THICK_FF) {
+    if (!ctx->gaunt_loaded) return fail(ctx, RTW_E_ARG, "free-free scene needs rtw_load_gaunt first");
+    d.ff_C = s.absorb * std::sqrt(FF_T0 / s.temperature);
+    // Gaunt column position for this temperature, Float32 as emission_functions.jl:58-66
+    double gam2 = GAM2_FACTOR / s.temperature;
+    float loggam = (float)std::log10(gam2);
+    float frac_j = (loggam + 6.0f) / 0.2f;
+    int fj = (int)std::floor(frac_j), cj = (int)std::ceil(frac_j + 1e-3f);
+    d.g_ok_j = (fj >= 1 && cj <= 81) ? 1 : 0;
+    d.gw0 = (float)cj - frac_j;
+    d.gw1 = frac_j - (float)fj;
+    std::vector<float2> cols(146);
+    for (int i = 0; i < 146; ++i) {
+      cols[i].x = d.g_ok_j ? ctx->gaunt[i * 81 + (fj - 1)] : 1.0f;
+      cols[i].y = d.g_ok_j ? ctx->gaunt[i * 81 + (cj - 1)] : 1.0f;
+    }
+    HIPCHK(ctx, ctx->gcols.ensure(146 * sizeof(float2)));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->gcols.p, cols.data(), 146 * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+  }
+  return RTW_OK;
+}
+
+// Wyman-Sloan-Shirley multi-lobe CIE fits with trapezoid weights
+// (main/color_theory.jl:11-61,79-91).
+static const double CIE_M[4][7] = {{0.362, 1.056, -0.065, 0.821, 0.286, 1.217, 0.681},
+                                   {442.0, 599.8, 501.1, 568.8, 530.9, 437.0, 459.0},
+                                   {0.0624, 0.0264, 0.0490, 0.0213, 0.0613, 0.0845, 0.0385},
+                                   {0.0374, 0.0323, 0.0382, 0.0247, 0.0322, 0.0278, 0.0725}};
+static double lobe(double wl, int i) {
+  double d = wl - CIE_M[1][i];
+  double sg = d >= 0 ? CIE_M[3][i] : CIE_M[2][i];
+  double t = d * sg;
+  return CIE_M[0][i] * std::exp(-0.5 * t * t);
+}
+
+static std::vector<DevColor> make_colors(const DevScene& S, const double* lam, int n) {
+  std::vector<DevColor> v(n);
+  double step = n > 1 ? lam[1] - lam[0] : 1.0;
+  for (int i = 0; i < n; ++i) {
+    DevColor& c = v[i];
+    c.f = 1.0 / lam[i];
+    c.uf = F_FACTOR * c.f / S.T;
+    c.lc = std::log10(c.uf);
+    double r = FF_NU0 / c.f;
+    c.if3 = r * r * r;
+    c.cx = c.f * c.f * c.f * C_LIGHT / (lam[i] * lam[i]);
+    double w = step;
+    if (i == 0) w /= 2;
+    if (i == n - 1) w /= 2;
+    c.wx = (lobe(lam[i], 0) + lobe(lam[i], 1) + lobe(lam[i], 2)) * w;
+    c.wy = (lobe(lam[i], 3) + lobe(lam[i], 4)) * w;
+    c.wz = (lobe(lam[i], 5) + lobe(lam[i], 6)) * w;
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void fill_u32(unsigned* p, int n, unsigned v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// maxfinite of Y over the pose (LttM.jl:183); non-negative doubles order like their bits.
+__global__ void max_y_kernel(const double* xyz, long long n, unsigned long long* out) {
+  unsigned long long best = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double X = xyz[3 * i], Y = xyz[3 * i + 1], Z = xyz[3 * i + 2];
+    double yy = (X != 0 && Y != 0 && Z != 0) ? Y : 0.0;  // calc_xyY black rule (color_theory.jl:103-107)
+    if (isfinite(yy) && yy > 0) {
+      unsigned long long b = (unsigned long long)__double_as_longlong(yy);
+      best = b > best ? b : best;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long t = __shfl_xor(best, o);
+    best = t > best ? t : best;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
+__device__ double srgb_c(double v) { return v <= 0.0031308 ? 12.92 * v : 1.055 * pow(v, 1 / 2.4) - 0.055; }
+
+// scaleminmax(0, maxY), xyY -> XYZ -> linear sRGB (IEC 61966-2-1) -> 16 bit, image
+// transpose(reverse(img, dims=2)) (LttM.jl:193-208).
+__global__ void srgb_kernel(const double* xyz, int nx, int ny, const unsigned long long* maxy, uint16_t* out) {
+  long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= (long long)nx * ny) return;
+  int i = (int)(p % nx), j = (int)(p / nx);
+  double X = xyz[3 * p], Y = xyz[3 * p + 1], Z = xyz[3 * p + 2];
+  double x = 1, y = 1, Yv = 0;
+  if (X != 0 && Y != 0 && Z != 0) {
+    double d = X + Y + Z;
+    x = X / d; y = Y / d; Yv = Y;
+  }
+  double m = __longlong_as_double((long long)*maxy);
+  double Ys = m > 0 ? fmin(fmax(Yv / m, 0.0), 1.0) : 0.0;
+  if (!(Ys == Ys)) Ys = 0;
+  double Xs = x * Ys / y, Zs = (1 - x - y) * Ys / y;
+  double c[3] = {3.2404542 * Xs - 1.5371385 * Ys - 0.4985314 * Zs, -0.9692660 * Xs + 1.8760108 * Ys + 0.0415560 * Zs,
+                 0.0556434 * Xs - 0.2040259 * Ys + 1.0572252 * Zs};
+  long long row = (long long)(ny - 1 - j);
+  for (int k = 0; k < 3; ++k) {
+    double v = fmin(1.0, fmax(0.0, srgb_c(c[k])));
+    if (!(v == v) || m <= 0) v = 0;
+    out[3 * (row * nx + i) + k] = (uint16_t)llrint(v * 65535.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int rtw_version(void) { return RTW_ABI_VERSION; }
+
+int rtw_context_create(int32_t device, rtw_ctx** out) {
+  if (!out) return fail(nullptr, RTW_E_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(nullptr, RTW_E_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(nullptr, RTW_E_ARG, "bad device index");
+  rtw_ctx* c = new rtw_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(nullptr, RTW_E_DEVICE, "stream/event creation failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  if (c->num_cu <= 0) c->num_cu = 256;
+  *out = c;
+  return RTW_OK;
+}
+
+void rtw_context_destroy(rtw_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* rtw_last_error(const rtw_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+// Parse gauntff_merged_Z01.dat as emission_functions.jl:15 loads it: skip 42 lines,
+// skip '#' lines, then 146 rows x 81 values, stored as Float32.
+int rtw_load_gaunt(rtw_ctx* c, const char* path) {
+  if (!c || !path) return fail(c, RTW_E_ARG, "NULL argument");
+  std::ifstream f(path);
+  if (!f) return fail(c, RTW_E_ARG, std::string("cannot open ") + path);
+  std::string line;
+  for (int i = 0; i < 42 && std::getline(f, line); ++i) {}
+  std::vector<float> t(146 * 81);
+  int row = 0;
+  while (row < 146 && std::getline(f, line)) {
+    size_t q = line.find_first_not_of(" \t");
+    if (q == std::string::npos || line[q] == '#') continue;
+    std::istringstream is(line);
+    for (int j = 0; j < 81; ++j) {
+      double v;
+      if (!(is >> v)) return fail(c, RTW_E_ARG, "Gaunt table: short row");
+      t[row * 81 + j] = (float)v;
+    }
+    ++row;
+  }
+  if (row != 146) return fail(c, RTW_E_ARG, "Gaunt table: expected 146 rows");
+  c->gaunt.swap(t);
+  c->gaunt_loaded = true;
+  return RTW_OK;
+}
+
+// Common set-up + launch.  Outputs are device pointers.
+static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams,
+                          const rtw_integrator* opts, const double* lam, int32_t n_lambda, const double* init_rays,
+                          double* d_xyz, uint8_t* d_cls, int32_t* d_steps, double* d_rays, hipStream_t st,
+                          rtw_stats* stats, bool timed) {
+  if (!c) return RTW_E_ARG;
+  if (!scene || !cams || n_cams < 1 || !opts || !lam || n_lambda < 0 || !d_xyz)
+    return fail(c, RTW_E_ARG, "NULL or invalid argument");
+  if (n_lambda > 64) return fail(c, RTW_E_UNSUPPORTED, "at most 64 wavelengths");
+  if (init_rays && n_cams != 1) return fail(c, RTW_E_ARG, "init_rays needs exactly one camera");
+  if (!(opts->rtol > 0) || !(opts->atol > 0) || !(opts->max_dt_scale > 0) || !(opts->max_steps >= 1))
+    return fail(c, RTW_E_ARG, "invalid tolerances/max_dt_scale/max_steps");
+  if (opts->precision != RTW_PREC_FP64 && opts->precision != RTW_PREC_MIXED)
+    return fail(c, RTW_E_ARG, "unknown precision");
+  for (int k = 0; k < n_lambda; ++k)
+    if (!(lam[k] > 0)) return fail(c, RTW_E_ARG, "wavelengths must be positive");
+  const rtw_camera& c0 = cams[0];
+  if (c0.x_pix < 1 || !(c0.fov_hor > 0 && c0.fov_hor < 180) || !(c0.fov_vert > 0 && c0.fov_vert < 180))
+    return fail(c, RTW_E_ARG, "invalid camera grid");
+  for (int k = 0; k < n_cams; ++k) {
+    if (cams[k].x_pix != c0.x_pix || cams[k].fov_hor != c0.fov_hor || cams[k].fov_vert != c0.fov_vert)
+      return fail(c, RTW_E_ARG, "all poses must share x_pix and field of view");
+    if (cam_is_singular(cams[k].pos)) return fail(c, RTW_E_CAMERA, "camera on a coordinate singularity");
+    if (!(std::fabs(cams[k].beta) < 1)) return fail(c, RTW_E_ARG, "|beta| must be < 1");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  DevScene S;
+  int rc = make_scene(c, *scene, *opts, S);
+  if (rc) return rc;
+  const int ny = cam_y_pix(c0);
+  if (ny < 1) return fail(c, RTW_E_ARG, "camera has no rows");
+  const long long npix = (long long)c0.x_pix * ny;
+  const long long total = npix * n_cams;
+  if (total >= (1LL << 31)) return fail(c, RTW_E_ARG, "too many rays in one call");
+
+  std::vector<DevCam> dc(n_cams);
+  for (int k = 0; k < n_cams; ++k) dc[k] = make_cam(S, cams[k]);
+  std::vector<DevColor> cols = make_colors(S, lam, n_lambda);
+  HIPCHK(c, c->cams.ensure(sizeof(DevCam) * n_cams));
+  HIPCHK(c, c->colors.ensure(sizeof(DevColor) * (n_lambda + 1)));
+  HIPCHK(c, c->queue.ensure(64));
+  HIPCHK(c, c->counters.ensure(64));
+  if (!c->gcols.p) HIPCHK(c, c->gcols.ensure(146 * sizeof(float2)));
+  HIPCHK(c, hipMemcpyAsync(c->cams.p, dc.data(), sizeof(DevCam) * n_cams, hipMemcpyHostToDevice, st));
+  if (n_lambda) HIPCHK(c, hipMemcpyAsync(c->colors.p, cols.data(), sizeof(DevColor) * n_lambda, hipMemcpyHostToDevice, st));
+  const double* d_init = nullptr;
+  if (init_rays) {
+    HIPCHK(c, c->init.ensure(sizeof(double) * npix * 8));
+    HIPCHK(c, hipMemcpyAsync(c->init.p, init_rays, sizeof(double) * npix * 8, hipMemcpyHostToDevice, st));
+    d_init = c->init.as<double>();
+  }
+  fill_u32<<<1, 64, 0, st>>>(c->queue.as<unsigned>(), 16, 0u);
+  fill_u64<<<1, 64, 0, st>>>(c->counters.as<unsigned long long>(), 8, 0ull);
+
+  KArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.S = S;
+  A.O.rtol = opts->rtol;
+  A.O.atol = opts->atol;
+  A.O.max_steps = opts->max_steps;
+  A.O.tau_thr = -std::log(opts->atol);
+  A.O.fp64_radius = opts->fp64_radius;
+  A.O.ncol = n_lambda;
+  A.O.n = 8 + 2 * n_lambda;
+  A.cams = c->cams.as<DevCam>();
+  A.x_pix = c0.x_pix;
+  A.y_pix = ny;
+  A.npix = npix;
+  A.total = total;
+  A.init_rays = d_init;
+  A.colors = c->colors.as<DevColor>();
+  A.gcols = c->gcols.as<float2>();
+  A.out_xyz = d_xyz;
+  A.out_cls = d_cls;
+  A.out_steps = d_steps;
+  A.out_rays = d_rays;
+  A.queue = c->queue.as<unsigned>();
+  A.counters = c->counters.as<unsigned long long>();
+  A.persistent = opts->schedule == RTW_SCHED_PERSISTENT ? 1 : 0;
+
+  HIPCHK(c, c->kargs.ensure(sizeof(KArgs)));
+  HIPCHK(c, hipMemcpyAsync(c->kargs.p, &A, sizeof(KArgs), hipMemcpyHostToDevice, st));
+  LaunchInfo li;
+  if (timed) HIPCHK(c, hipEventRecord(c->ev0, st));
+  hipError_t e = launch_rk45(A, c->kargs.as<KArgs>(), opts->precision == RTW_PREC_MIXED, opts->lanes_per_ray, c->num_cu, st, li);
+  if (e != hipSuccess) return fail(c, RTW_E_DEVICE, std::string("integrator launch: ") + hipGetErrorString(e) + " " + li.msg);
+  if (timed) HIPCHK(c, hipEventRecord(c->ev1, st));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->rays = total;
+    stats->lanes_per_ray = li.lanes;
+  }
+  return RTW_OK;
+}
+
+static int collect_stats(rtw_ctx* c, rtw_stats* stats, bool timed) {
+  if (!stats) return RTW_OK;
+  unsigned long long cnt[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  stats->steps_attempted = (int64_t)cnt[0];
+  stats->steps_accepted = (int64_t)cnt[1];
+  stats->steps_fp64 = (int64_t)cnt[2];
+  if (timed) {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->kernel_ms = ms;
+  }
+  return RTW_OK;
+}
+
+int rtw_render(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams, const rtw_integrator* opts,
+               const double* lambda_nm, int32_t n_lambda, const double* init_rays, double* out_xyz, uint8_t* out_class,
+               int32_t* out_steps, double* out_rays, rtw_stats* stats) {
+  if (!c) return RTW_E_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!out_xyz || !cams || n_cams < 1) return fail(c, RTW_E_ARG, "NULL or invalid argument");
+  int ny = cam_y_pix(cams[0]);
+  long long total = (long long)cams[0].x_pix * (ny > 0 ? ny : 0) * n_cams;
+  int n = 8 + 2 * (n_lambda > 0 ? n_lambda : 0);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, c->xyz.ensure(sizeof(double) * 3 * (total > 0 ? total : 1)));
+  if (out_class) HIPCHK(c, c->cls.ensure(total > 0 ? total : 1));
+  if (out_steps) HIPCHK(c, c->steps.ensure(sizeof(int32_t) * 2 * (total > 0 ? total : 1)));
+  if (out_rays) HIPCHK(c, c->rays.ensure(sizeof(double) * n * (total > 0 ? total : 1)));
+  int rc = render_enqueue(c, scene, cams, n_cams, opts, lambda_nm, n_lambda, init_rays, c->xyz.as<double>(),
+                          out_class ? c->cls.as<uint8_t>() : nullptr, out_steps ? c->steps.as<int32_t>() : nullptr,
+                          out_rays ? c->rays.as<double>() : nullptr, c->stream, stats, true);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out_xyz, c->xyz.p, sizeof(double) * 3 * total, hipMemcpyDeviceToHost));
+  if (out_class) HIPCHK(c, hipMemcpy(out_class, c->cls.p, total, hipMemcpyDeviceToHost));
+  if (out_steps) HIPCHK(c, hipMemcpy(out_steps, c->steps.p, sizeof(int32_t) * 2 * total, hipMemcpyDeviceToHost));
+  if (out_rays) HIPCHK(c, hipMemcpy(out_rays, c->rays.p, sizeof(double) * n * total, hipMemcpyDeviceToHost));
+  rc = collect_stats(c, stats, true);
+  if (rc) return rc;
+  if (stats)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RTW_OK;
+}
+
+int rtw_render_dev(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams, const rtw_integrator* opts,
+                   const double* lambda_nm, int32_t n_lambda, double* d_xyz, uint8_t* d_class, int32_t* d_steps,
+                   void* stream) {
+  if (!c) return RTW_E_ARG;
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  return render_enqueue(c, scene, cams, n_cams, opts, lambda_nm, n_lambda, nullptr, d_xyz, d_class, d_steps, nullptr, st,
+                        nullptr, false);
+}
+
+int rtw_xyz_to_srgb16(rtw_ctx* c, const double* xyz, int32_t x_pix, int32_t y_pix, uint16_t* out_rgb16) {
+  if (!c) return RTW_E_ARG;
+  if (!xyz || !out_rgb16 || x_pix < 1 || y_pix < 1) return fail(c, RTW_E_ARG, "NULL or invalid argument");
+  long long n = (long long)x_pix * y_pix;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, c->aux.ensure(sizeof(double) * 3 * n + 64 + sizeof(uint16_t) * 3 * n));
+  char* base = c->aux.as<char>();
+  double* dx = reinterpret_cast<double*>(base);
+  unsigned long long* dm = reinterpret_cast<unsigned long long*>(base + sizeof(double) * 3 * n);
+  uint16_t* dout = reinterpret_cast<uint16_t*>(base + sizeof(double) * 3 * n + 64);
+  HIPCHK(c, hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  fill_u64<<<1, 64, 0, c->stream>>>(dm, 1, 0ull);
+  max_y_kernel<<<256, 256, 0, c->stream>>>(dx, n, dm);
+  srgb_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(dx, x_pix, y_pix, dm, dout);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out_rgb16, dout, sizeof(uint16_t) * 3 * n, hipMemcpyDeviceToHost));
+  return RTW_OK;
+}
+
+int rtw_render_adjoint(rtw_ctx* c, const rtw_scene*, const rtw_camera*, const rtw_integrator*, const double*, int32_t,
+                       const double*, double*, double*, rtw_stats*) {
+  return fail(c, RTW_E_UNSUPPORTED, "adjoint not built");
+}
+
+}  // extern "C"

@@ -1,0 +1,342 @@
+// rtw_device.h -- scene functions of the gfx950 integrator (device side).
+//
+// Each RTWCA scene directory (main/*.source) defines a fixed function set that
+// CUDA.jl compiles into the ray kernel.  Here every scene is a compile-time KIND
+// with the same set: geodesic right-hand side (Christoffel contraction), metric,
+// fluid 4-velocity and frequency shift (the "stage scalars" the transfer needs),
+// step cap, pole handling and bounds.  Functions are templated on the stage
+// scalar type T (double for the plain method, float for the mixed-precision
+// stages); step-level logic (pad_max_dt, near_singularity, keepinbounds, the
+// controller) is always double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtw {
+
+constexpr double PI = 3.141592653589793;
+constexpr double NO_DIV_ZERO = 1e-24;                      // main/64BitConstants.jl:21
+constexpr double H_PLANCK = 6.62607015e-34;                // :13
+constexpr double C_LIGHT = 299792458.0;                    // :17
+constexpr double K_B = 1.380649e-23;                       // :19
+constexpr double EMISSION_SCALE = 2 * H_PLANCK * C_LIGHT * 1e27;  // :23
+constexpr double F_FACTOR = H_PLANCK * C_LIGHT * 1e9 / K_B;       // :25
+constexpr double GAM2_FACTOR = 2.1798723611035e-18 / K_B;         // Ry / k_B
+constexpr double FF_T0 = 1e4;           // thick disk opacity normalisation temperature
+constexpr double FF_NU0 = 1.0 / 550.0;  // and frequency (nm^-1)
+
+enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5 };
+enum { CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
+
+// Scene constants, derived on the host from rtw_scene.
+struct DevScene {
+  int kind, flags;
+  double M, a, rs, r_ev;
+  double T, absorb, density;
+  double r_in, r_out, th_lo, th_hi;
+  double cos_cut;      // thick disk: |cos theta| <= z_cut * H/r
+  double inv_2h2;      // 1 / (2 (H/r)^2)
+  double r_escape;
+  double r_horizon;    // (1 + 10 rtol) r_ev; 0 without horizon
+  double r_photon15;   // 1.5 * 3M: Schwarzschild step-cap radius
+  double eps_main, eps_pole, eps_photon;  // pad_max_dt epsilons
+  double bb_a;         // blackbody a_nu
+  double ff_C;         // kappa * sqrt(T0 / T)
+  float gw0, gw1;      // Gaunt column weights (ceil_j - frac_j), (frac_j - floor_j)
+  int g_ok_j;          // Gaunt column indices in range
+  int ref_f32;         // Float32 Christoffel symbols
+};
+
+struct DevOpts {
+  double rtol, atol, max_steps, tau_thr, fp64_radius;
+  int ncol, n;
+};
+
+// Per-colour constants (staged in LDS).
+struct DevColor {
+  double f;     // 1 / lambda (nm^-1)
+  double uf;    // F_FACTOR f / T  (u = uf * shift)
+  double lc;    // log10(uf)
+  double if3;   // (nu0 / f)^3
+  double cx;    // f^3 c / lambda^2: I_lambda = (I/nu^3) * cx  (color_theory.jl:73)
+  double wx, wy, wz;  // CIE weights (color_theory.jl:79-91)
+};
+
+// Camera pose: per-render transform e R2 L R and pixel-grid constants.
+struct DevCam {
+  double Mc[16];
+  double pos[4];
+  double W, H, rho_h, rho_v;
+};
+
+template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
+
+__device__ __forceinline__ void sc_(double x, double& s, double& c) { sincos(x, &s, &c); }
+__device__ __forceinline__ void sc_(float x, float& s, float& c) { sincosf(x, &s, &c); }
+__device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float rsq_(float x) { return 1.0f / sqrtf(x); }
+__device__ __forceinline__ double exp_(double x) { return exp(x); }
+__device__ __forceinline__ float exp_(float x) { return __expf(x); }
+__device__ __forceinline__ double log10_(double x) { return log10(x); }
+__device__ __forceinline__ float log10_(float x) { return __log10f(x); }
+
+// Stage output of the geodesic right-hand side.
+template <class T> struct StageOut {
+  T d[8];      // dx/dlambda
+  T sh;        // frequency shift -u_j g_ij k^i (gen_geometry.jl:373-390)
+  T w;         // thick disk: rho^2 at the point
+  bool matter; // is_fire
+};
+
+// Kerr Christoffel symbols contracted with k (Kerr.source/source_geometry.jl:195-243,
+// symmetric pairs summed once).  Gt is the type the symbols are evaluated in.
+template <class Gt, class T>
+__device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4], T acc[4]) {
+  Gt s, c;
+  sc_(th, s, c);
+  Gt r2 = r * r, a2 = a * a, s2 = s * s, c2 = c * c;
+  Gt ac2 = a2 * c2, as2 = a2 * s2;
+  Gt S = r2 + ac2;
+  Gt D = r2 - r * rs + a2;
+  Gt Aa = (r2 + a2) * S + rs * r * as2;
+  Gt s2t = Gt(2) * s * c;
+  Gt iS = Gt(1) / S, iD = Gt(1) / D, is = Gt(1) / s;
+  Gt iS2 = iS * iS, iS3 = iS2 * iS;
+  Gt cot = c * is;
+  Gt rma = r2 - ac2;
+  Gt h2S2D = Gt(0.5) * iS2 * iD;
+  // t
+  Gt Gt_tr = rs * (r2 + a2) * rma * h2S2D;
+  Gt Gt_tth = -Gt(0.5) * rs * a2 * r * s2t * iS2;
+  Gt Gt_rph = rs * a * s2 * (ac2 * (a2 - r2) - r2 * (a2 + Gt(3) * r2)) * h2S2D;
+  Gt Gt_thph = rs * a2 * a * r * s2 * s * c * iS2;
+  // r
+  Gt Gr_tt = Gt(0.5) * rs * D * (r2 - ac2) * iS3;
+  Gt Gr_tph = -Gt(0.5) * D * rs * a * s2 * rma * iS3;
+  Gt Gr_rr = Gt(0.5) * (Gt(2) * r * as2 - rs * rma) * iS * iD;
+  Gt Gr_rth = -Gt(0.5) * a2 * s2t * iS;
+  Gt Gr_thth = -r * D * iS;
+  Gt Gr_phph = Gt(0.5) * D * s2 * (-Gt(2) * r * S * S + rs * as2 * rma) * iS3;
+  // theta
+  Gt Gth_tt = -Gt(0.5) * rs * a2 * r * s2t * iS3;
+  Gt Gth_tph = Gt(0.5) * rs * a * r * (r2 + a2) * s2t * iS3;
+  Gt Gth_rr = Gt(0.5) * a2 * s2t * iS * iD;
+  Gt Gth_rth = r * iS;
+  Gt Gth_thth = -Gt(0.5) * a2 * s2t * iS;
+  Gt Gth_phph = -Gt(0.5) * s2t * (Aa * S + (r2 + a2) * rs * r * as2) * iS3;
+  // phi
+  Gt Gp_tr = rs * a * rma * h2S2D;
+  Gt Gp_tth = -rs * a * r * cot * iS2;
+  Gt hs = Gt(0.5) * a2 * s2t;
+  Gt Gp_rph = (Gt(2) * r * S * S + rs * (hs * hs - r2 * (S + r2 + a2))) * h2S2D;
+  Gt Gp_thph = cot * (S * S + rs * r * as2) * iS2;
+
+  T kt = k[0], kr = k[1], kth = k[2], kph = k[3];
+  T tt = kt * kt, tr = kt * kr, tth = kt * kth, tph = kt * kph;
+  T rr = kr * kr, rth = kr * kth, rph = kr * kph, thth = kth * kth, thph = kth * kph, phph = kph * kph;
+  acc[0] = T(-2) * (T(Gt_tr) * tr + T(Gt_tth) * tth + T(Gt_rph) * rph + T(Gt_thph) * thph);
+  acc[1] = -(T(Gr_tt) * tt + T(2) * T(Gr_tph) * tph + T(Gr_rr) * rr + T(2) * T(Gr_rth) * rth + T(Gr_thth) * thth +
+             T(Gr_phph) * phph);
+  acc[2] = -(T(Gth_tt) * tt + T(2) * T(Gth_tph) * tph + T(Gth_rr) * rr + T(2) * T(Gth_rth) * rth +
+             T(Gth_thth) * thth + T(Gth_phph) * phph);
+  acc[3] = T(-2) * (T(Gp_tr) * tr + T(Gp_tth) * tth + T(Gp_rph) * rph + T(Gp_thph) * thph);
+}
+
+// Geodesic right-hand side and stage scalars at x (calc_ray_derivative!,
+// main/Iterators/LttM/integrators_LttM.jl:11-36).
+template <int KIND, class T>
+__device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOut<T>& o) {
+  const T r = x[1], th = x[2];
+  const T kt = x[4], kr = x[5], kth = x[6], kph = x[7];
+  o.d[0] = kt; o.d[1] = kr; o.d[2] = kth; o.d[3] = kph;
+  o.w = T(0);
+  if constexpr (KIND == MINK) {
+    // Minkowski_Sphere_Spherical.source/source_geometry.jl:93-108
+    T s, c;
+    sc_(th, s, c);
+    T ir = T(1) / (r + T(NO_DIV_ZERO));
+    T cot = c / s;
+    o.d[4] = T(0);
+    o.d[5] = r * kth * kth + r * s * s * kph * kph;
+    o.d[6] = T(-2) * ir * kr * kth + s * c * kph * kph;
+    o.d[7] = T(-2) * (ir * kr * kph + cot * kth * kph);
+    // metric (-1,1,r^2,...), static source: shift = k^t (source_model.jl:288-294)
+    o.sh = kt;
+    o.matter = r <= T(S.r_out);
+  } else if constexpr (KIND == SCHW) {
+    // Schwarzschild_Simple_Disk_Spherical.source/source_geometry.jl:109-128
+    T s, c;
+    sc_(th, s, c);
+    const T rs = T(S.rs);
+    T ir = T(1) / r;
+    T rmrs = r - rs;
+    T Gt_tr = rs * T(0.5) * ir / rmrs;
+    T Gr_tt = T(0.5) * rs * rmrs * ir * ir * ir;
+    T Gr_rr = -Gt_tr;
+    T cot = c / s;
+    o.d[4] = T(-2) * Gt_tr * kt * kr;
+    o.d[5] = -(Gr_tt * kt * kt + Gr_rr * kr * kr - rmrs * kth * kth - rmrs * s * s * kph * kph);
+    o.d[6] = -(T(2) * ir * kr * kth - s * c * kph * kph);
+    o.d[7] = T(-2) * (ir * kr * kph + cot * kth * kph);
+    // metric and source velocity (source_model.jl:312-333)
+    T gtt = -(T(1) - rs * ir);
+    T gpp = sq(r * s);
+    bool m = T(S.r_in) <= r && r <= T(S.r_out) && T(S.th_lo) <= th && th <= T(S.th_hi);
+    o.matter = m;
+    const T M = T(S.M);
+    if (m) {
+      T ut = rsq_(T(1) - T(3) * M * ir);
+      T up = ut * sqrt(M * ir * ir * ir);
+      o.sh = -(ut * (gtt * kt) + up * (gpp * kph));
+    } else {
+      T ut = rsq_(T(1) - T(2) * M * ir);
+      o.sh = -(ut * (gtt * kt));
+    }
+  } else {
+    // Kerr.source/source_geometry.jl:55-75 (metric), :195-243 (symbols)
+    const T k4[4] = {kt, kr, kth, kph};
+    T acc[4];
+    if constexpr (sizeof(T) == 8) {
+      if (S.ref_f32)
+        kerr_accel<float, T>((float)r, (float)th, (float)S.a, (float)S.rs, k4, acc);
+      else
+        kerr_accel<T, T>(r, th, T(S.a), T(S.rs), k4, acc);
+    } else {
+      kerr_accel<T, T>(r, th, T(S.a), T(S.rs), k4, acc);
+    }
+    o.d[4] = acc[0]; o.d[5] = acc[1]; o.d[6] = acc[2]; o.d[7] = acc[3];
+    T s, c;
+    sc_(th, s, c);
+    const T a = T(S.a), rs = T(S.rs), M = T(S.M);
+    T r2 = r * r, a2 = a * a, s2 = s * s;
+    T Sg = r2 + a2 * c * c;
+    T D = r2 - r * rs + a2;
+    T A = (r2 + a2) * Sg + rs * r * a2 * s2;
+    T iSg = T(1) / Sg;
+    T gtt = -(T(1) - rs * r * iSg);
+    T gtp = -rs * a * r * s2 * iSg;
+    T gpp = A * iSg * s2;
+    bool m;
+    if constexpr (KIND == KERR_FF) {
+      m = T(S.r_in) <= r && r <= T(S.r_out) && fabs(c) <= T(S.cos_cut);
+      if (m) {
+        T rho = T(S.density) * exp_(-(c * c) * T(S.inv_2h2));
+        o.w = rho * rho;
+      }
+    } else {
+      m = T(S.r_in) <= r && r <= T(S.r_out) && T(S.th_lo) <= th && th <= T(S.th_hi);
+    }
+    o.matter = m;
+    T ut, Om;
+    if (m) {  // prograde Keplerian orbit (Kerr.source/source_model.jl:110-120)
+      T sr = sqrt(r);
+      T r15 = r * sr;
+      T sM = sqrt(M);
+      Om = sM / (r15 + a * sM);
+      T Dk = r15 - T(3) * M * sr + T(2) * a * sM;
+      ut = (r15 + a * sM) / (sqrt(r15) * sqrt(Dk));
+    } else {  // FIDO (:122-134)
+      ut = sqrt(A * iSg / D);
+      Om = T(2) * a * M * r / A;
+    }
+    T up = ut * Om;
+    o.sh = -(ut * (gtt * kt + gtp * kph) + up * (gtp * kt + gpp * kph));
+  }
+}
+
+// pad_max_dt (Minkowski source_model.jl:266-282, Schwarzschild :286-306, Kerr :86-103).
+template <int KIND>
+__device__ __forceinline__ double pad_max_dt(const DevScene& S, const double* x) {
+  const double d = NO_DIV_ZERO;
+  double eps;
+  if (x[2] < PI / 8 || x[2] > 7 * PI / 8) eps = S.eps_pole;
+  else if (KIND == SCHW && x[1] <= S.r_photon15) eps = S.eps_photon;
+  else eps = S.eps_main;
+  double A = (KIND >= KERR) ? fabs(1.0 / (eps / fabs(x[5])) + d) : fabs(1.0 / (eps / (fabs(x[5]) + d)) + d);
+  double th = x[2];
+  double mth = (PI - th < th) ? PI - th : th;
+  double B = fabs(1.0 / (eps * mth / (fabs(x[6]) + d) + d));
+  double C = fabs(1.0 / (eps / (fabs(x[7]) + d) + d));
+  double m = 1.0 / (A + B + C + d);
+  m = (0.5 < m) ? 0.5 : m;
+  const double tiny = (double)1e-12f;
+  m = (m < tiny) ? tiny : m;
+  return m;
+}
+
+// near_singularity (Minkowski source_geometry.jl:114-132; Schwarzschild :134-154;
+// Kerr :249-269).  Returns true and the Euler step when close to a pole (or, for
+// Minkowski, the origin).
+template <int KIND>
+__device__ __forceinline__ bool near_singularity(const double* x, double h, double atol, double& ts) {
+  const double d = NO_DIV_ZERO;
+  double th_new = x[2] + h * x[6];
+  double new_r = x[1] + h * x[5];
+  if (KIND == MINK) {
+    double lim = 10 * atol / fabs(new_r + d);
+    if (fabs(th_new) <= lim) { ts = 2 * fabs(x[2]) / (fabs(x[6] * 37) + d); return true; }
+    if (fabs(th_new - PI) <= lim) { ts = 2 * fabs((x[2] - PI) / (fabs(x[6] * 37) + d)); return true; }
+    if (fabs(new_r) <= 100 * atol) { ts = 2 * atol / (fabs(x[5] * 37) + d); return true; }
+    return false;
+  }
+  if (fabs(th_new) <= atol) { ts = 2 * fabs(x[2]) / (fabs(x[6] * 37) + d); return true; }
+  if (fabs(th_new - PI) <= atol) { ts = 2 * fabs((x[2] - PI) / (fabs(x[6] * 37) + d)); return true; }
+  return false;
+}
+
+// Julia mod(x, 2pi) for x >= 0 (floored; keepinbounds! only calls it after theta >= 0).
+__device__ __forceinline__ double jmod(double x, double y) {
+  double r = fmod(x, y);
+  if (r == 0.0) return copysign(0.0, y);
+  if ((r > 0) != (y > 0)) return r + y;
+  return r;
+}
+
+// keepinbounds! (Minkowski source_geometry.jl:138-163; Kerr :275-301).
+template <int KIND> __device__ __forceinline__ void keepinbounds(double* x) {
+  if (KIND == MINK && x[1] <= 0) {
+    x[1] = -x[1] + NO_DIV_ZERO;
+    x[5] = -x[5];
+    x[2] -= PI;
+    x[6] = -x[6];
+  }
+  if (x[2] < 0) {
+    x[2] = -x[2];
+    x[3] -= PI;
+    x[6] = -x[6];
+  }
+  x[2] = jmod(x[2], 2 * PI);
+  if (x[2] >= PI) {
+    x[2] = 2 * PI - x[2];
+    x[3] += PI;
+    x[6] = -x[6];
+  }
+}
+
+// van Hoof Gaunt lookup (main/emission_functions.jl:54-78) with the row index from
+// log10(u) and the two columns for the scene temperature staged in LDS as
+// float2 (col floor_j, col ceil_j) per row.  Float32 as the reference, written
+// with round-to-nearest intrinsics so no FMA contraction changes the result.
+__device__ __forceinline__ float gaunt_row_lookup(float logu, const float2* col, float w0, float w1, int ok_j) {
+  float frac_i = __fdiv_rn(__fadd_rn(logu, 16.0f), 0.2f);
+  int fi = (int)floorf(frac_i);
+  int ci = (int)ceilf(__fadd_rn(frac_i, 1e-3f));
+  if (ok_j && fi >= 1 && ci <= 146) {
+    float2 lo = col[fi - 1], hi = col[ci - 1];
+    float a = __fsub_rn((float)ci, frac_i), b = __fsub_rn(frac_i, (float)fi);
+    float g1 = __fadd_rn(__fmul_rn(a, lo.x), __fmul_rn(b, hi.x));
+    float g2 = __fadd_rn(__fmul_rn(a, lo.y), __fmul_rn(b, hi.y));
+    return __fadd_rn(__fmul_rn(w0, g1), __fmul_rn(w1, g2));
+  }
+  return 1.2f;
+}
+
+// Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
+struct StageSc {
+  double sh;   // shift
+  double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4
+  double lg;   // free-free: log10(shift)
+  double vac;  // 1 if no matter at this stage
+};
+
+}  // namespace rtw

@@ -1,0 +1,591 @@
+// rtw_rk45.h -- the per-pixel adaptive Dormand-Prince RK45 integrator for gfx950.
+//
+// Semantics: RKDP_Step_w_buffer! and integrate_ray
+// (main/Iterators/LttM/integrators_LttM.jl:54-224, main/Iterators/LttM/LttM.jl:11-53),
+// including the averaged-slope FSAL (the next step's k1 is sum(b_i k_i)), the
+// Euler fallback near the poles, the error norm over all 8 + 2N components and
+// the controller branches.
+//
+// Layout: a ray is integrated by a group of L lanes of one wavefront
+// (L = 8: 8 rays per wave64).  Lane l owns
+//   * geodesic components j with j % L == l (one component when L = 8), and
+//   * colours c = l + m*L, m < CMAX, i.e. (I/nu^3, tau) and their last slopes.
+// The geodesic state is also kept replicated in every lane; each RK stage is a
+// gather of the owned stage components (8 ds_bpermute) followed by a replicated
+// evaluation of the geodesic right-hand side, which also yields the per-stage
+// transfer scalars (frequency shift, matter flag, density) that the colour
+// lanes read from LDS.  The transfer equations of one colour depend only on
+// those scalars and on tau, so each lane integrates its colours stage by stage
+// with forward-accumulated stage sums (the same left-to-right summation order
+// as the reference).  Candidate spectral states are staged in LDS and committed
+// only when the step is accepted; the error norm is a group reduction.
+//
+// Scheduling: SCHED_GRID gives each group one ray; SCHED_PERSISTENT keeps
+// groups resident and lets a group that finishes its ray pull the next one from
+// a device counter in the same loop iteration, so a wave keeps stepping L-lane
+// groups with live rays however uneven the step counts are.
+#pragma once
+#include "rtw_device.h"
+
+namespace rtw {
+
+constexpr int BLOCK = 256;
+
+// Dormand-Prince tableau (main/64BitConstants.jl:28-37) in forward-accumulation
+// form: DPC[s][t] multiplies k_s into the sum of stage s+t (stage 7 = the b row),
+// DPE[s] is the error weight b*_s (row 9), DPB[s] the b row.
+__constant__ double DPC[8][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0, 1.0 / 5, 3.0 / 40, 44.0 / 45, 19372.0 / 6561, 9017.0 / 3168},      // k1 -> stages 2..6
+    {0, 9.0 / 40, -56.0 / 15, -25360.0 / 2187, -355.0 / 33, 0},            // k2 -> 3..7 (b2 = 0)
+    {0, 32.0 / 9, 64448.0 / 6561, 46732.0 / 5247, 500.0 / 1113, 0},        // k3 -> 4..7
+    {0, -212.0 / 729, 49.0 / 176, 125.0 / 192, 0, 0},                      // k4 -> 5..7
+    {0, -5103.0 / 18656, -2187.0 / 6784, 0, 0, 0},                         // k5 -> 6..7
+    {0, 11.0 / 84, 0, 0, 0, 0},                                            // k6 -> 7
+    {0, 0, 0, 0, 0, 0}};
+__constant__ double DPE[8] = {0, 5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100,
+                              1.0 / 40};
+constexpr double B1 = 35.0 / 384;
+
+// k1 contributions to stages 2..7 (slot t <-> stage 2 + t; stage 7 is b1).
+__device__ __forceinline__ double k1c(int t) {
+  switch (t) {
+    case 0: return 1.0 / 5;
+    case 1: return 3.0 / 40;
+    case 2: return 44.0 / 45;
+    case 3: return 19372.0 / 6561;
+    case 4: return 9017.0 / 3168;
+    default: return B1;
+  }
+}
+
+struct KArgs {
+  DevScene S;
+  DevOpts O;
+  const DevCam* cams;
+  int x_pix, y_pix;
+  long long npix;   // pixels per pose
+  long long total;  // rays in the launch
+  const double* init_rays;  // optional (npix, 8) column-major, single pose
+  const DevColor* colors;
+  const float2* gcols;      // 146 rows of the two Gaunt columns
+  double* out_xyz;
+  uint8_t* out_cls;
+  int32_t* out_steps;
+  double* out_rays;         // optional (total, n) column-major
+  unsigned* queue;
+  unsigned long long* counters;  // attempted, accepted, fp64
+  int persistent;
+};
+
+// v[j] for a lane-dependent j without dynamic register indexing: an OR of masked
+// bit patterns (a select chain would be folded back into an indexed load, which
+// forces the array into scratch memory).
+__device__ __forceinline__ double sel8(const double* v, int j) {
+  unsigned long long r = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    unsigned long long m = 0ull - (unsigned long long)(j == k);
+    r |= (unsigned long long)__double_as_longlong(v[k]) & m;
+  }
+  return __longlong_as_double((long long)r);
+}
+
+template <int L> __device__ __forceinline__ double gsum(double v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+  return v;
+}
+template <int L> __device__ __forceinline__ int gand(int v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v &= __shfl_xor(v, o, L);
+  return v;
+}
+
+// Stage scalars of the transfer equation from a geodesic stage.
+template <int KIND, class T>
+__device__ __forceinline__ StageSc make_stage(const DevScene& S, const StageOut<T>& o) {
+  StageSc ss;
+  double sh = (double)o.sh;
+  ss.sh = sh;
+  ss.lg = 0;
+  if (!o.matter) {
+    ss.vac = 1;
+    ss.P = (-1.0 / sh) * 0.0;
+  } else {
+    ss.vac = 0;
+    if constexpr (KIND == KERR_FF) {
+      T q = T(-1) / o.sh;
+      ss.P = (double)(q * T(S.ff_C) * o.w / (o.sh * o.sh * o.sh));
+      ss.lg = (double)log10_(o.sh);
+    } else {
+      ss.P = (-1.0 / sh) * S.bb_a;
+    }
+  }
+  return ss;
+}
+
+// Transfer slopes of one colour at one stage (integrators_LttM.jl:37-42 with the
+// scene's calc_radiative_coefficients).  Evaluated in T.
+template <int KIND, class T>
+__device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, const DevColor& c, double tau,
+                                          const float2* gcol, double& kT, double& kI) {
+  if (ss.vac != 0) {
+    kT = ss.P;
+    kI = ss.P;
+    return;
+  }
+  const T ES = T(EMISSION_SCALE);
+  if constexpr (KIND == KERR_FF) {
+    // a = C rho^2 (nu0/nu)^3 (1 - e^-u) g_ff,  j = a B_nu(T)  (Kirchhoff)
+    T u = T(c.uf) * T(ss.sh);
+    T em = exp_(-u);
+    float g = gaunt_row_lookup((float)(c.lc + ss.lg), gcol, S.gw0, S.gw1, S.g_ok_j);
+    T base = T(ss.P) * T(c.if3) * T(g);
+    kT = (double)(base * (T(1) - em));
+    kI = (double)(base * ES * exp_(-u - T(tau)));
+  } else {
+    // a = const, j = a B_nu(T)
+    T P = T(ss.P);
+    kT = (double)P;
+    kI = (double)(P * ES * exp_(-T(tau)) / (exp_(T(c.uf) * T(ss.sh)) - T(1)));
+  }
+}
+
+template <int KIND, int L, int CMAX, bool MIXED>
+struct Integrator {
+  static constexpr int GM = 8 / L;  // geodesic components per lane
+
+  const KArgs& A;   // in global memory (read through the scalar cache)
+  const DevScene& S;
+  int lane;
+  int tid;
+  const DevColor* scol;  // LDS
+  const float2* sg;      // LDS
+  StageSc* sst;          // LDS, 6 stage slots of this group
+  double* cand;          // LDS candidate area (CMAX*4 per lane, stride BLOCK)
+
+  // ray state
+  double Rf[8];       // replicated geodesic state
+  double Ro[GM];      // owned components of Rf
+  double LS[GM];      // owned components of the last slope (k1)
+  double sI[CMAX], sT[CMAX], lI[CMAX], lT[CMAX];
+  double h;
+  bool rejected;
+
+  __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, StageSc* st, double* cd)
+      : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sst(st), cand(cd) {}
+
+  __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < A.O.ncol; }
+  __device__ __forceinline__ void sync_own() {
+#pragma unroll
+    for (int g = 0; g < GM; ++g) Ro[g] = sel8(Rf, lane + g * L);
+  }
+  __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 4 + q) * BLOCK + tid]; }
+
+  __device__ __forceinline__ void gather(const double* xo, double* X) const {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) X[j] = __shfl(xo[j / L], j % L, L);
+  }
+
+  // Full derivative at the current state: owned geodesic slope + owned colours'
+  // slopes (calc_ray_derivative!, integrators_LttM.jl:9-48).
+  template <class T> __device__ __forceinline__ void full_deriv(double* dfull) {
+    T x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = (T)Rf[j];
+    StageOut<T> o;
+    geo_rhs<KIND, T>(S, x, o);
+    StageSc ss = make_stage<KIND, T>(S, o);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dfull[j] = (double)o.d[j];
+#pragma unroll
+    for (int g = 0; g < GM; ++g) LS[g] = sel8(dfull, lane + g * L);
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) {
+      if (col_ok(m)) spec_eval<KIND, T>(S, ss, scol[lane + m * L], sT[m], sg, lT[m], lI[m]);
+    }
+  }
+
+  __device__ __forceinline__ bool all_thick() const {
+    int t = 1;
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m)
+      if (col_ok(m) && sT[m] < A.O.tau_thr) t = 0;
+    return gand<L>(t) != 0;
+  }
+
+  // calc_terminate (Kerr.source/source_model.jl:168-182 and the other scenes).
+  __device__ __forceinline__ bool terminate(double max_steps, double step_count) const {
+    if (step_count >= max_steps || Rf[1] > S.r_escape || (KIND != MINK && Rf[1] <= S.r_horizon) ||
+        Rf[3] > 8 * PI)
+      return true;
+    return all_thick();
+  }
+
+  __device__ __forceinline__ int classify() const {
+    bool thick = all_thick();
+    if (!isfinite(Rf[1])) return CLS_BOUNDARY;
+    if (KIND != MINK && Rf[1] <= S.r_horizon) return CLS_HORIZON;
+    if (Rf[1] > S.r_escape) return CLS_ESCAPE;
+    if (thick && A.O.ncol > 0) return CLS_OPAQUE;
+    return CLS_BOUNDARY;
+  }
+
+  __device__ __forceinline__ void deriv_any(bool f64, double* d) {
+    if constexpr (MIXED) {
+      if (f64) full_deriv<double>(d); else full_deriv<float>(d);
+    } else {
+      full_deriv<double>(d);
+    }
+  }
+
+  // Start a ray: initial state, h = -pad_max_dt, one derivative (LttM.jl:24-40, 70-71).
+  __device__ __forceinline__ void start(long long ray) {
+    long long pose = ray / A.npix;
+    long long pix = ray - pose * A.npix;
+    if (A.init_rays) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) Rf[k] = A.init_rays[pix + k * A.npix];
+    } else {
+      const DevCam& C = A.cams[pose];
+      int i = (int)(pix % A.x_pix), j = (int)(pix / A.x_pix);
+      double w = C.rho_h * ((i + 1) - 0.5) - C.W;
+      double hh = C.rho_v * ((j + 1) - 0.5) - C.H;
+      double th = atan(sqrt(w * w + hh * hh));
+      double ph;
+      if (w > 0) ph = atan(hh / w);
+      else if (w < 0 && hh >= 0) ph = atan(hh / w) + PI;
+      else if (w < 0 && hh < 0) ph = atan(hh / w) - PI;
+      else if (w == 0 && hh > 0) ph = PI / 2;
+      else if (w == 0 && hh < 0) ph = -PI / 2;
+      else ph = 0;
+      double st, ct, sp, cp;
+      sincos(th, &st, &ct);
+      sincos(ph, &sp, &cp);
+      double v[4] = {1.0, -st * cp, -st * sp, -ct};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Rf[k] = C.pos[k];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) s += C.Mc[a * 4 + b] * v[b];
+        Rf[4 + a] = s;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) { sI[m] = 0; sT[m] = 0; lI[m] = 0; lT[m] = 0; }
+    sync_own();
+    h = -pad_max_dt<KIND>(S, Rf);
+    double d[8];
+    deriv_any(!MIXED || Rf[1] < A.O.fp64_radius, d);
+    rejected = false;
+  }
+
+  // Euler fallback (integrators_LttM.jl:62-105).  Returns true if the ray
+  // terminated inside it (the step then counts as accepted, h unchanged).
+  __device__ __forceinline__ bool euler(double ts, bool f64) {
+    double lsf[8];
+    gather(LS, lsf);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Rf[j] -= ts * lsf[j];
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) { sI[m] -= ts * lI[m]; sT[m] -= ts * lT[m]; }
+    keepinbounds<KIND>(Rf);
+    // 36 Euler substeps, then one more derivative for the RK attempt; a single
+    // derivative call site keeps the code compact.
+#pragma unroll 1
+    for (int it = 0; it <= 36; ++it) {
+      sync_own();
+      const bool term = terminate(2, 1);
+      if (term) {
+        if (it == 36) return true;
+        continue;
+      }
+      double d[8];
+      deriv_any(f64, d);
+      if (it == 36) break;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Rf[j] -= ts * d[j];
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) { sI[m] -= ts * lI[m]; sT[m] -= ts * lT[m]; }
+      keepinbounds<KIND>(Rf);
+    }
+    h = -pad_max_dt<KIND>(S, Rf);
+    return false;
+  }
+
+  // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
+  __device__ __forceinline__ bool step(bool& used64) {
+    const double atol = A.O.atol, rtol = A.O.rtol;
+    const double max_dt = pad_max_dt<KIND>(S, Rf);
+    const bool f64 = !MIXED || Rf[1] < A.O.fp64_radius;
+    used64 = f64;
+    double ts;
+    if (near_singularity<KIND>(Rf, h, atol, ts)) {
+      if (euler(ts, f64)) return true;  // h unchanged, accepted
+    }
+    const double hh = h;
+
+    // ---- geodesic stages 2..7 ------------------------------------------------
+    double acc[GM][6], eg[GM], ns[GM];
+#pragma unroll
+    for (int g = 0; g < GM; ++g) {
+#pragma unroll
+      for (int t = 0; t < 6; ++t) acc[g][t] = k1c(t) * LS[g];
+      eg[g] = DPE[1] * LS[g];
+    }
+    double X7[8];
+    int vac_all = 1;
+#pragma unroll 1
+    for (int s = 2; s <= 7; ++s) {
+      double xo[GM];
+#pragma unroll
+      for (int g = 0; g < GM; ++g) {
+        xo[g] = Ro[g] + hh * acc[g][0];
+        if (s == 7) ns[g] = acc[g][0];
+      }
+      double X[8];
+      gather(xo, X);
+      double K[8];
+      StageSc ss;
+      if (!MIXED || f64) {
+        StageOut<double> o;
+        geo_rhs<KIND, double>(S, X, o);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) K[j] = o.d[j];
+        ss = make_stage<KIND, double>(S, o);
+      } else {
+        float xf[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xf[j] = (float)X[j];
+        StageOut<float> o;
+        geo_rhs<KIND, float>(S, xf, o);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) K[j] = (double)o.d[j];
+        ss = make_stage<KIND, float>(S, o);
+      }
+      sst[s - 2] = ss;  // identical in every lane of the group
+      vac_all &= (ss.vac != 0 && ss.P == 0.0) ? 1 : 0;
+#pragma unroll
+      for (int g = 0; g < GM; ++g) {
+        double k = sel8(K, lane + g * L);
+        eg[g] += DPE[s] * k;
+#pragma unroll
+        for (int t = 1; t < 6; ++t) acc[g][t] += DPC[s][t] * k;
+#pragma unroll
+        for (int t = 0; t < 5; ++t) acc[g][t] = acc[g][t + 1];
+        acc[g][5] = 0;
+      }
+      if (s == 7) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) X7[j] = X[j];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // error of the owned geodesic components; y = X7 (same expression)
+    double e2 = 0;
+#pragma unroll
+    for (int g = 0; g < GM; ++g) {
+      double y = sel8(X7, lane + g * L);
+      double dl = y - Ro[g] - hh * eg[g];
+      double tol = atol + fmax(fabs(Ro[g]), fabs(y)) * rtol;
+      double q = dl / tol;
+      e2 += q * q;
+    }
+
+    // ---- transfer equations of the owned colours -------------------------------
+    if (vac_all) {
+      // every stage 2..7 in vacuum: k_s = 0, only k1 contributes
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) {
+        if (!col_ok(m)) continue;
+        double nT = lT[m] * B1, nI = lI[m] * B1;
+        double yT = sT[m] + hh * nT, yI = sI[m] + hh * nI;
+        double dT = (yT - sT[m]) - hh * (lT[m] * DPE[1]);
+        double dI = (yI - sI[m]) - hh * (lI[m] * DPE[1]);
+        double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
+        double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
+        e2 += qI * qI + qT * qT;
+        CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) {
+        if (!col_ok(m)) continue;
+        const DevColor c = scol[lane + m * L];
+        double aT[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) aT[t] = k1c(t) * lT[m];
+        double nI = lI[m] * B1, eT = lT[m] * DPE[1], eI = lI[m] * DPE[1];
+        double nT = 0, yT = 0;
+#pragma unroll 1
+        for (int s = 2; s <= 7; ++s) {
+          double tau = sT[m] + hh * aT[0];
+          if (s == 7) { nT = aT[0]; yT = tau; }
+          const StageSc ss = sst[s - 2];
+          double kT, kI;
+          if (!MIXED || f64) spec_eval<KIND, double>(S, ss, c, tau, sg, kT, kI);
+          else spec_eval<KIND, float>(S, ss, c, tau, sg, kT, kI);
+          eT += DPE[s] * kT;
+          eI += DPE[s] * kI;
+          if (s <= 6) nI += DPC[s][7 - s] * kI;
+#pragma unroll
+          for (int t = 1; t < 6; ++t) aT[t] += DPC[s][t] * kT;
+#pragma unroll
+          for (int t = 0; t < 5; ++t) aT[t] = aT[t + 1];
+          aT[5] = 0;
+        }
+        double yI = sI[m] + hh * nI;
+        double dT = yT - sT[m] - hh * eT;
+        double dI = yI - sI[m] - hh * eI;
+        double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
+        double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
+        e2 += qI * qI + qT * qT;
+        CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+      }
+    }
+
+    // ---- error norm and controller (integrators_LttM.jl:179-223) ----------------
+    double err = sqrt(gsum<L>(e2) / (double)A.O.n);
+    double nh = 0.9 * hh * pow(1.0 / err, 0.2);
+    if (fabs(nh) > max_dt) nh = -max_dt;
+    bool accept;
+    if (err == 0 || !isfinite(nh)) {
+      accept = true;
+      nh = hh;
+    } else if (err > 1) {
+      accept = false;
+      if (rejected && fabs(nh) >= fabs(hh)) nh = hh / 2;
+      keepinbounds<KIND>(Rf);
+      sync_own();
+    } else {
+      accept = true;
+      if (fabs(nh) > 5 * fabs(hh)) nh = 5 * hh;
+    }
+    if (accept) {
+      keepinbounds<KIND>(X7);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Rf[j] = X7[j];
+      sync_own();
+#pragma unroll
+      for (int g = 0; g < GM; ++g) LS[g] = ns[g];
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) {
+        if (!col_ok(m)) continue;
+        sI[m] = CAND(m, 0); sT[m] = CAND(m, 1); lI[m] = CAND(m, 2); lT[m] = CAND(m, 3);
+      }
+    }
+    h = nh;
+    rejected = !accept;
+    return accept;
+  }
+
+  // calc_xyY sums (color_theory.jl:67-102) and the outputs of one ray.
+  __device__ __forceinline__ void finish(long long ray, int attempted, int accepted) {
+    double X = 0, Y = 0, Z = 0;
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) {
+      if (!col_ok(m)) continue;
+      const DevColor& c = scol[lane + m * L];
+      double I = sI[m] * c.cx;
+      X += I * c.wx; Y += I * c.wy; Z += I * c.wz;
+    }
+    X = gsum<L>(X); Y = gsum<L>(Y); Z = gsum<L>(Z);
+    int cls = classify();
+    if (lane == 0) {
+      if (A.out_xyz) {
+        A.out_xyz[3 * ray + 0] = X;
+        A.out_xyz[3 * ray + 1] = Y;
+        A.out_xyz[3 * ray + 2] = Z;
+      }
+      if (A.out_cls) A.out_cls[ray] = (uint8_t)cls;
+      if (A.out_steps) {
+        A.out_steps[2 * ray] = attempted;
+        A.out_steps[2 * ray + 1] = accepted;
+      }
+    }
+    if (A.out_rays) {
+      long long T = A.total;
+#pragma unroll
+      for (int g = 0; g < GM; ++g) A.out_rays[ray + (long long)(lane + g * L) * T] = Ro[g];
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) {
+        if (!col_ok(m)) continue;
+        int c = lane + m * L;
+        A.out_rays[ray + (long long)(8 + 2 * c) * T] = sI[m];
+        A.out_rays[ray + (long long)(9 + 2 * c) * T] = sT[m];
+      }
+    }
+  }
+};
+
+template <int KIND, int L, int CMAX, bool MIXED>
+__global__ void __launch_bounds__(BLOCK) rk45_kernel(const KArgs* __restrict__ Ap) {
+  const KArgs& A = *Ap;
+  extern __shared__ double smem[];
+  const int ncol = A.O.ncol;
+  DevColor* scol = reinterpret_cast<DevColor*>(smem);
+  const int ncol_pad = (ncol + 1) & ~1;
+  float2* sg = reinterpret_cast<float2*>(scol + ncol_pad);
+  StageSc* sstage = reinterpret_cast<StageSc*>(sg + 146);
+  double* cand = reinterpret_cast<double*>(sstage + (BLOCK / L) * 6);
+  for (int i = threadIdx.x; i < ncol; i += BLOCK) scol[i] = A.colors[i];
+  if (KIND == KERR_FF)
+    for (int i = threadIdx.x; i < 146; i += BLOCK) sg[i] = A.gcols[i];
+  __syncthreads();
+
+  const int lane = threadIdx.x % L;
+  const int grp = threadIdx.x / L;
+  Integrator<KIND, L, CMAX, MIXED> it(A, lane, threadIdx.x, scol, sg, sstage + grp * 6, cand);
+
+  const long long groups = (long long)gridDim.x * (BLOCK / L);
+  long long ray = -1;
+  long long next_static = (long long)blockIdx.x * (BLOCK / L) + grp;
+  bool need = true;
+  int attempted = 0, accepted = 0;
+  unsigned long long tot_att = 0, tot_acc = 0, tot_64 = 0;
+  const double max_steps = A.O.max_steps;
+  while (true) {
+    if (need) {
+      if (A.persistent) {
+        unsigned v = 0;
+        if (lane == 0) v = atomicAdd(A.queue, 1u);
+        ray = (long long)(unsigned)__shfl((int)v, 0, L);
+      } else {
+        ray = next_static;
+        next_static += groups;
+      }
+      if (ray >= A.total) break;
+      it.start(ray);
+      attempted = 0;
+      accepted = 0;
+      need = false;
+    }
+    bool used64;
+    bool acc = it.step(used64);
+    attempted++;
+    accepted += acc ? 1 : 0;
+    tot_64 += used64 ? 1 : 0;
+    if (it.terminate(max_steps, (double)attempted)) {
+      it.finish(ray, attempted, accepted);
+      tot_att += attempted;
+      tot_acc += accepted;
+      need = true;
+    }
+  }
+  if (lane == 0 && A.counters) {
+    atomicAdd(&A.counters[0], tot_att);
+    atomicAdd(&A.counters[1], tot_acc);
+    atomicAdd(&A.counters[2], tot_64);
+  }
+}
+
+template <int L, int CMAX> constexpr size_t rk45_lds_bytes(int ncol) {
+  return sizeof(DevColor) * (size_t)((ncol + 1) & ~1) + sizeof(float2) * 146 + sizeof(StageSc) * (BLOCK / L) * 6 +
+         sizeof(double) * (size_t)CMAX * 4 * BLOCK;
+}
+
+}  // namespace rtw

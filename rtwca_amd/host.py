@@ -1,0 +1,163 @@
+"""Python host side of librtw: the reference's integrator-mode interface.
+
+`gen_image` has the keyword arguments of RTWCA's gen_image
+(main/Iterators/LttM/LttM.jl:104-107, main/Iterators/5P/5P.jl:135-138) and returns
+the image as a (y_pix, x_pix, 3) uint16 array (the reference's Matrix{RGB{N0f16}})
+or (image, rays) when returnrays=True, with rays shaped (x_pix, y_pix, n) like the
+reference.  The scene replaces the `include`d *.source directory (see
+rtwca_amd.scenes.initialize_world).
+
+All integration happens in librtw.so on the GPU; if the library or a GPU is
+missing, the calls raise -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtw.so")
+GAUNT_PATH = os.path.join(HERE, "data", "gauntff_merged_Z01.dat")
+
+_lib = None
+
+
+class RTWError(RuntimeError):
+    pass
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RTWError("librtw.so is not built (run `python -m rtwca_amd.build`)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.rtw_version.restype = C.c_int
+    L.rtw_context_create.argtypes = [C.c_int32, C.POINTER(P)]
+    L.rtw_context_destroy.argtypes = [P]
+    L.rtw_context_destroy.restype = None
+    L.rtw_last_error.argtypes = [P]
+    L.rtw_last_error.restype = C.c_char_p
+    L.rtw_load_gaunt.argtypes = [P, C.c_char_p]
+    L.rtw_render.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
+    L.rtw_render_dev.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P]
+    L.rtw_xyz_to_srgb16.argtypes = [P, P, C.c_int32, C.c_int32, P]
+    L.rtw_render_adjoint.argtypes = [P, P, P, P, P, C.c_int32, P, P, P, P]
+    _lib = L
+    return L
+
+
+def lib():
+    return _load()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _ref(x):
+    return C.cast(C.byref(x), C.c_void_p)
+
+
+class Context:
+    """A librtw context bound to one GPU (rtw_context_create)."""
+
+    def __init__(self, device=0):
+        L = _load()
+        h = C.c_void_p()
+        rc = L.rtw_context_create(int(device), C.byref(h))
+        if rc != 0:
+            raise RTWError("rtw_context_create: %d %s" % (rc, L.rtw_last_error(None).decode()))
+        self.h = h
+        self.L = L
+        rc = L.rtw_load_gaunt(self.h, GAUNT_PATH.encode())
+        self._check(rc, "rtw_load_gaunt")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RTWError("%s failed (%d): %s" % (what, rc, self.L.rtw_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h:
+            self.L.rtw_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, scene, cameras, integ, lam, init_rays=None, want_rays=False, want_class=True,
+               want_steps=True):
+        """rtw_render: returns dict(xyz (npix,3), cls (npix,), steps (npix,2), rays (npix,n), stats)."""
+        if isinstance(cameras, abi.Camera):
+            cameras = [cameras]
+        ncam = len(cameras)
+        cams = (abi.Camera * ncam)(*cameras)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        c0 = cameras[0]
+        W = np.tan(np.pi * c0.fov_hor / 180 / 2)
+        H = np.tan(np.pi * c0.fov_vert / 180 / 2)
+        ny = int(np.rint(c0.x_pix * H / W))
+        npix = c0.x_pix * ny * ncam
+        n = 8 + 2 * len(lam)
+        xyz = np.zeros((npix, 3))
+        cls = np.zeros(npix, dtype=np.uint8) if want_class else None
+        steps = np.zeros((npix, 2), dtype=np.int32) if want_steps else None
+        rays = np.zeros(npix * n) if want_rays else None
+        ir = None
+        if init_rays is not None:
+            ir = np.ascontiguousarray(np.asarray(init_rays, dtype=np.float64).T.reshape(-1))
+        st = abi.Stats()
+        rc = self.L.rtw_render(self.h, _ref(scene), C.cast(cams, C.c_void_p), ncam, _ref(integ), _ptr(lam), len(lam),
+                               _ptr(ir), _ptr(xyz), _ptr(cls), _ptr(steps), _ptr(rays), _ref(st))
+        self._check(rc, "rtw_render")
+        return {"xyz": xyz, "cls": cls, "steps": steps,
+                "rays": None if rays is None else rays.reshape(n, npix).T.copy(),
+                "stats": {k: getattr(st, k) for k, _ in abi.Stats._fields_ if k != "reserved"},
+                "x_pix": c0.x_pix, "y_pix": ny}
+
+    def xyz_to_srgb16(self, xyz, x_pix, y_pix):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        out = np.zeros((y_pix, x_pix, 3), dtype=np.uint16)
+        rc = self.L.rtw_xyz_to_srgb16(self.h, _ptr(xyz), int(x_pix), int(y_pix), _ptr(out))
+        self._check(rc, "rtw_xyz_to_srgb16")
+        return out
+
+
+_ctx = None
+
+
+def context():
+    global _ctx
+    if _ctx is None:
+        _ctx = Context(0)
+    return _ctx
+
+
+def gen_image(scene, camera_pos=(0, 0, 0, 0), camera_dir=(0.0, 0.0), speed=0.0, camera_point=(0.0, 0.0, 0.0),
+              x_pix=100, max_steps=1e3, colors=(400, 550, 700), returnrays=False, tolerance=1e-4,
+              max_dt_scale=1e-1, fov_hor=85, fov_vert=60, print_num_pix=False, ray_bundles=False,
+              abs_tol=None, rel_tol=None, precision=abi.PREC_FP64, schedule=abi.SCHED_PERSISTENT):
+    """gen_image with the reference's keywords (LttM.jl:104-107).  `tolerance` fills both
+    tolerances as the reference does (LttM.jl:131-132); abs_tol/rel_tol override them."""
+    if ray_bundles:
+        raise RTWError("ray bundles (skybox anti-aliasing) are not provided by rtwca_amd")
+    cam = abi.make_camera(camera_pos, camera_dir, speed, camera_point, x_pix, fov_hor, fov_vert)
+    integ = abi.make_integrator(tolerance if rel_tol is None else rel_tol,
+                                tolerance if abs_tol is None else abs_tol, max_dt_scale, max_steps,
+                                precision, schedule)
+    ctx = context()
+    r = ctx.render(scene, [cam], integ, list(colors), want_rays=returnrays)
+    if print_num_pix:
+        print("Number of Pixels ", r["x_pix"] * r["y_pix"])
+    img = ctx.xyz_to_srgb16(r["xyz"], r["x_pix"], r["y_pix"])
+    if returnrays:
+        rays = r["rays"].reshape(r["y_pix"], r["x_pix"], -1).transpose(1, 0, 2)
+        return img, rays
+    return img

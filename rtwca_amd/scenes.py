@@ -1,0 +1,193 @@
+"""Scene presets: the reference's initialize_world cases and the benchmark configs.
+
+`initialize_world(name)` mirrors main/initialization.jl:78-172: it returns the
+scene (instead of `include`-ing a *.source directory) and the default camera
+arguments (position, direction, pointing, beta) for that scene.
+
+`config(n)` returns the five benchmark workloads (scene, camera, integrator,
+wavelengths).  Values that the reference does not fix for these new scenes are
+decided here, once:
+  * cfg1 sphere: absorption a_nu = 100 per unit length, so the unit sphere is
+    optically thick (tau >= -log(atol) = 23 within a chord of 0.23);
+  * cfg2 disk: the reference's theta wedge [15pi/32, 17pi/32] with a_nu = 1;
+  * cfg3/4 thick disk: r from the reference-formula ISCO to 12, H = 0.1 r,
+    truncated at |z| <= 4H, kappa = 0.2, tolerance rtol = atol = 1e-6 (cfg3),
+    1e-8 (cfg4);
+  * cfg5 torus + jet: see `synch_scene`.
+"""
+import math
+from fractions import Fraction
+
+from . import abi
+
+PI = math.pi
+
+
+def wavelengths(lo=380, hi=780, n=64):
+    """range(lo, hi, length=n) as Julia computes it (correctly rounded nodes)."""
+    if n == 1:
+        return [float(lo)]
+    step = Fraction(hi - lo, n - 1)
+    return [float(Fraction(lo) + step * k) for k in range(n)]
+
+
+def f32(x):
+    import struct
+    return struct.unpack("f", struct.pack("f", x))[0]
+
+
+def kerr_isco_ref(a, M=1.0, f32_mode=False):
+    """calc_orb_limits prograde ISCO (main/Kerr.source/source_geometry.jl:20-26),
+    including its (1 +- chi)^(3/2) exponent."""
+    if f32_mode:
+        import numpy as np
+        af, Mf = np.float32(a), np.float32(M)
+        rs = np.float32(2) * Mf
+        chi = np.float32(2) * af / rs
+        Z1 = 1.0 + float(np.float32(1) - chi * chi) ** (1 / 3) * (float(np.float32(1) + chi) ** 1.5 + float(np.float32(1) - chi) ** 1.5)
+        Z2 = math.sqrt(float(np.float32(3) * (chi * chi)) + Z1 * Z1)
+        return float(np.float32(float(Mf) * (3 + Z2 - math.sqrt((3 - Z1) * (3 + Z1 + 2 * Z2)))))
+    rs = 2 * M
+    chi = 2 * a / rs
+    Z1 = 1 + (1 - chi * chi) ** (1 / 3) * ((1 + chi) ** 1.5 + (1 - chi) ** 1.5)
+    Z2 = math.sqrt(3 * (chi * chi) + Z1 * Z1)
+    return M * (3 + Z2 - math.sqrt((3 - Z1) * (3 + Z1 + 2 * Z2)))
+
+
+def kerr_circ_beta(r, a, M=1.0):
+    """calc_circ_geod_beta (main/Kerr.source/source_geometry.jl:44-49)."""
+    rs = 2 * M
+    D = r * r - r * rs + a * a
+    return math.sqrt(M) * (r * r - 2 * a * math.sqrt(M * r) + a * a) / (math.sqrt(D) * (r ** 1.5 + a * math.sqrt(M)))
+
+
+def _base(kind):
+    s = abi.Scene()
+    s.kind = kind
+    s.flags = 0
+    s.mass = 1.0
+    s.spin = 0.0
+    s.temperature = 5778.0
+    s.absorb = 1.0
+    s.density = 1.0
+    s.r_in = 0.0
+    s.r_out = 1.0
+    s.th_lo = 0.0
+    s.th_hi = PI
+    s.h_over_r = 0.1
+    s.z_cut = 4.0
+    s.r_escape = 100.0
+    s.bfield = 10.0
+    s.theta_e = 10.0
+    s.jet_half_angle = 0.0
+    s.r_jet = 0.0
+    s.torus_r_max = 12.0
+    return s
+
+
+def minkowski_sphere(temperature=5778.0, absorb=1.0, radius=1.0):
+    """main/Minkowski_Sphere_Spherical.source: emitting ball r <= 1, a_nu = 1, T = 5778 K
+    (source_model.jl:224,240,255)."""
+    s = _base(abi.SCENE_MINKOWSKI_SPHERE)
+    s.temperature = temperature
+    s.absorb = absorb
+    s.r_out = radius
+    return s
+
+
+def schwarzschild_disk(temperature=5778.0, absorb=1.0, r_in=6.0, r_out=12.0):
+    """main/Schwarzschild_Simple_Disk_Spherical.source: wedge 6 <= r <= 12,
+    15pi/32 <= theta <= 17pi/32 (source_model.jl:273-280), a_nu = 1, T = 5778 K."""
+    s = _base(abi.SCENE_SCHWARZSCHILD_DISK)
+    s.temperature = temperature
+    s.absorb = absorb
+    s.r_in, s.r_out = r_in, r_out
+    s.th_lo, s.th_hi = 15 * PI / 32, 17 * PI / 32
+    return s
+
+
+def kerr_disk(spin=0.998, ref_f32=True):
+    """main/Kerr.source: alpha = 0.998f0 (source_geometry.jl:13), wedge from the
+    reference-formula ISCO to r = 12, 31pi/64 <= theta <= 33pi/64 (source_model.jl:73-80),
+    a_nu = 1f-1, T = 5778 K.  ref_f32 keeps the Float32 evaluation of the reference."""
+    s = _base(abi.SCENE_KERR_DISK)
+    s.flags = abi.SCENE_REF_F32 if ref_f32 else 0
+    s.spin = f32(spin) if ref_f32 else spin
+    s.absorb = f32(0.1) if ref_f32 else 0.1
+    s.r_in = kerr_isco_ref(spin, 1.0, ref_f32)
+    s.r_out = 12.0
+    s.th_lo, s.th_hi = 31 * PI / 64, 33 * PI / 64
+    return s
+
+
+def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, r_out=12.0,
+                  h_over_r=0.1, z_cut=4.0):
+    """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Gaussian thick
+    disk rho = rho_s exp(-z^2 / 2H^2), H = 0.1 r, |z| <= 4H, radiating free-free with the
+    van Hoof Gaunt factor (main/emission_functions.jl:54-78) and Kirchhoff's law."""
+    s = _base(abi.SCENE_KERR_THICK_FF)
+    s.spin = spin
+    s.temperature = temperature
+    s.density = density
+    s.absorb = kappa
+    s.r_in = kerr_isco_ref(spin) if r_in is None else r_in
+    s.r_out = r_out
+    s.h_over_r = h_over_r
+    s.z_cut = z_cut
+    return s
+
+
+def initialize_world(name):
+    """main/initialization.jl:78-172 (scenes available natively)."""
+    n = name.lower()
+    if n == "schwarzschild_t,r,θ,ϕ_simpledisk":
+        return schwarzschild_disk(), [0, 25, PI / 3, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
+    if n == "kerr_boyer_lindquist":
+        return kerr_disk(), [0, 25, 0.52 * PI, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
+    if n == "minkowski_t,r,θ,ϕ_sphere":
+        return minkowski_sphere(), [0, 4, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
+    raise ValueError("scene %r is not provided by rtwca_amd" % name)
+
+
+class Workload:
+    def __init__(self, name, scene, cameras, integ, lam):
+        self.name = name
+        self.scene = scene
+        self.cameras = cameras
+        self.integ = integ
+        self.lam = lam
+
+    @property
+    def x_pix(self):
+        return self.cameras[0].x_pix
+
+
+def config(idx, x_pix=None, precision=abi.PREC_FP64, schedule=abi.SCHED_PERSISTENT, n_lambda=64):
+    """Benchmark workloads 1..5 (see the module docstring for the choices made)."""
+    lam = wavelengths(380, 780, n_lambda)
+    if idx == 1:
+        sc = minkowski_sphere(temperature=5772.0, absorb=100.0)
+        cam = abi.make_camera([0, 10, PI / 2, 0], [PI / 2, 0], 0.0, [0, 0, -PI / 2],
+                              x_pix or 256, 60, 60)
+        it = abi.make_integrator(1e-8, 1e-10, 0.1, 1e4, precision, schedule)
+        return Workload("minkowski_sphere_256", sc, [cam], it, lam)
+    if idx == 2:
+        sc = schwarzschild_disk(temperature=1e4, absorb=1.0, r_in=6.0, r_out=20.0)
+        cam = abi.make_camera([0, 30, 80 * PI / 180, 0], [PI / 2, 0], 0.0, [0, 0, -PI / 2],
+                              x_pix or 512, 40, 40)
+        it = abi.make_integrator(1e-8, 1e-10, 0.1, 1e4, precision, schedule)
+        return Workload("schwarzschild_disk_512", sc, [cam], it, lam)
+    if idx == 3:
+        sc = kerr_thick_ff(spin=0.9)
+        beta = kerr_circ_beta(5.0, 0.9)
+        cam = abi.make_camera([0, 5, PI / 2, 0], [0, 0], beta, [-PI / 2, 0, 0],
+                              x_pix or 1024, 90, 90)
+        it = abi.make_integrator(1e-6, 1e-6, 0.1, 1e4, precision, schedule, 3.0)
+        return Workload("kerr_a0.9_thick_ff_1024", sc, [cam], it, lam)
+    if idx == 4:
+        sc = kerr_thick_ff(spin=0.9, r_in=kerr_isco_ref(0.9))
+        cam = abi.make_camera([0, 10, 60 * PI / 180, 0], [0, 0], 0.0, [-PI / 2, 0, 0],
+                              x_pix or 256, 60, 60)
+        it = abi.make_integrator(1e-8, 1e-8, 0.1, 1e4, precision, schedule)
+        return Workload("kerr_thick_ff_adjoint_256", sc, [cam], it, lam)
+    raise ValueError("unknown config %r" % idx)
