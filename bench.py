@@ -1,0 +1,158 @@
+"""Benchmark: rays/s and RK45 steps/s of the Kerr a=0.9 thick-disk 1024x1024 render.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--x-pix 1024] [--no-cpu]
+
+One "step" of this benchmark is one full render of the workload (every ray of the
+1024x1024 camera integrated to termination with 64 wavelengths) by the
+mixed-precision persistent kernel, camera rays generated on the device and the
+per-pixel XYZ left in HBM.  Timed with HIP events on the launch stream.  Besides
+the headline line it renders once with the plain FP64 kernel for the speed-up and
+the per-pixel XYZ error of the mixed-precision image.
+
+cpu_baseline: the oracle (CPU FP64 restatement of the reference integrator) on a
+fixed pixel sample, on the host's cores, threaded like LttM.jl:156-165.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32 (AMD published)
+PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--x-pix", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--schedule", type=int, default=1)
+    return ap.parse_args()
+
+
+def flops_per_step():
+    with open(os.path.join(ROOT, "tests", "golden", "flops_per_step.json")) as f:
+        return json.load(f)["cfg3"]["flops_per_step"]
+
+
+def gpu_render(ctx, w, precision, steps, warmup, torch):
+    """Timed renders through rtw_render_dev; returns (ms per render list, xyz, cls, steps tensor)."""
+    from rtwca_amd import abi, host
+    cams = (abi.Camera * len(w.cameras))(*w.cameras)
+    integ = abi.make_integrator(w.integ.rtol, w.integ.atol, w.integ.max_dt_scale, w.integ.max_steps,
+                                precision, w.integ.schedule, w.integ.fp64_radius)
+    lam = np.ascontiguousarray(w.lam, dtype=np.float64)
+    c0 = w.cameras[0]
+    ny = int(np.rint(c0.x_pix * np.tan(np.pi * c0.fov_vert / 360) / np.tan(np.pi * c0.fov_hor / 360)))
+    npix = c0.x_pix * ny * len(w.cameras)
+    dev = torch.device("cuda:0")
+    xyz = torch.empty((npix, 3), dtype=torch.float64, device=dev)
+    cls = torch.empty((npix,), dtype=torch.uint8, device=dev)
+    stp = torch.empty((npix, 2), dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream(device=dev)  # default priority; the library launches on it
+    L = host.lib()
+
+    def once():
+        rc = L.rtw_render_dev(ctx.h, C.cast(C.byref(w.scene), C.c_void_p), C.cast(cams, C.c_void_p), len(w.cameras),
+                              C.cast(C.byref(integ), C.c_void_p), lam.ctypes.data_as(C.c_void_p), len(lam),
+                              C.c_void_p(xyz.data_ptr()), C.c_void_p(cls.data_ptr()), C.c_void_p(stp.data_ptr()),
+                              C.c_void_p(stream.cuda_stream))
+        ctx._check(rc, "rtw_render_dev")
+
+    for _ in range(warmup):
+        once()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        once()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+        print("  render (%s) %.1f ms" % ("mixed" if precision else "fp64", times[-1]), file=sys.stderr, flush=True)
+    return times, xyz, cls, stp
+
+
+def cpu_baseline(w, sample_n):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    c0 = w.cameras[0]
+    npix = c0.x_pix * c0.x_pix
+    stride = max(1, npix // sample_n)
+    subset = np.arange(0, npix, stride)[:sample_n]
+    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nthreads = max(1, min(nthreads, 16))
+    r = O.render(w.scene, c0, w.integ, w.lam, nthreads=nthreads, subset=subset)
+    steps = int(r["steps"][subset, 0].sum())
+    return {"value": len(subset) / r["seconds"], "unit": "rays/s", "cores": nthreads, "kind": "port",
+            "steps_per_s": steps / r["seconds"], "seconds": r["seconds"],
+            "sample": "%d pixels of the 1024x1024 camera (every %dth), FP64 oracle, %d threads" %
+                      (len(subset), stride, nthreads)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from rtwca_amd import abi, host, scenes
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    w = scenes.config(3, x_pix=a.x_pix, precision=abi.PREC_MIXED, schedule=a.schedule)
+    ctx = host.Context(0)
+    times, xyz, cls, stp = gpu_render(ctx, w, abi.PREC_MIXED, a.steps, a.warmup, torch)
+    ms = float(np.mean(times))
+    npix = xyz.shape[0]
+    att = int(stp[:, 0].to(torch.int64).sum().item())
+    F = flops_per_step()
+    achieved = att * F / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "rays/s and RK45 steps/s, 1024^2 Kerr; mixed-prec vs FP64 at <=1e-3 XYZ error",
+        "value": npix / (ms * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 stages + f64 state/controller", "data": "synthetic",
+        "config": {"workload": w.name, "rays": npix, "n_lambda": len(w.lam), "rtol": w.integ.rtol,
+                   "atol": w.integ.atol, "max_dt_scale": w.integ.max_dt_scale, "max_steps": w.integ.max_steps,
+                   "precision": "mixed", "schedule": "persistent" if a.schedule else "grid"},
+        "rk45_steps_per_s": att / (ms * 1e-3),
+        "steps_attempted": att,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VALU, "traffic": None,
+                     "flops_per_step": F},
+    }
+    if not a.no_fp64:
+        t64, xyz64, cls64, stp64 = gpu_render(ctx, w, abi.PREC_FP64, 1, 0, torch)
+        ms64 = float(np.mean(t64))
+        att64 = int(stp64[:, 0].to(torch.int64).sum().item())
+        X, X64 = xyz.cpu().numpy(), xyz64.cpu().numpy()
+        ref = np.abs(X64).max(axis=1)
+        m = ref > 0
+        rel = np.abs(X - X64).max(axis=1)[m] / ref[m]
+        out["fp64"] = {"rays_per_s": npix / (ms64 * 1e-3), "ms": ms64, "rk45_steps_per_s": att64 / (ms64 * 1e-3),
+                       "achieved_tflops": att64 * F / (ms64 * 1e-3) / 1e12}
+        out["speedup_vs_fp64"] = ms64 / ms
+        out["xyz_rel_err_median"] = float(np.median(rel)) if m.any() else 0.0
+        out["xyz_rel_err_p99"] = float(np.percentile(rel, 99)) if m.any() else 0.0
+        out["class_agreement"] = float((cls.cpu().numpy() == cls64.cpu().numpy()).mean())
+    if not a.no_cpu:
+        cb = cpu_baseline(scenes.config(3, x_pix=a.x_pix), a.cpu_sample)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

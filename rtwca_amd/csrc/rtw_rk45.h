@@ -152,6 +152,10 @@ __device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, 
   }
 }
 
+#ifndef RTW_WAVES_PER_EU
+#define RTW_WAVES_PER_EU 2
+#endif
+
 template <int KIND, int L, int CMAX, bool MIXED>
 struct Integrator {
   static constexpr int GM = 8 / L;  // geodesic components per lane
@@ -165,10 +169,11 @@ struct Integrator {
   StageSc* sst;          // LDS, 6 stage slots of this group
   double* cand;          // LDS candidate area (CMAX*4 per lane, stride BLOCK)
 
-  // ray state
-  double Rf[8];       // replicated geodesic state
-  double Ro[GM];      // owned components of Rf
+  // Ray state.  Only the owned geodesic components persist across a step (the full
+  // vector is gathered where a step needs it), plus r and phi for the stop test.
+  double Ro[GM];      // owned components of [t r theta phi k^t k^r k^theta k^phi]
   double LS[GM];      // owned components of the last slope (k1)
+  double rr, ph;      // r and phi of the state
   double sI[CMAX], sT[CMAX], lI[CMAX], lT[CMAX];
   double h;
   bool rejected;
@@ -177,23 +182,26 @@ struct Integrator {
       : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sst(st), cand(cd) {}
 
   __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < A.O.ncol; }
-  __device__ __forceinline__ void sync_own() {
-#pragma unroll
-    for (int g = 0; g < GM; ++g) Ro[g] = sel8(Rf, lane + g * L);
-  }
   __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 4 + q) * BLOCK + tid]; }
 
   __device__ __forceinline__ void gather(const double* xo, double* X) const {
 #pragma unroll
     for (int j = 0; j < 8; ++j) X[j] = __shfl(xo[j / L], j % L, L);
   }
+  // Keep the owned components of a full (replicated) state.
+  __device__ __forceinline__ void own(const double* X) {
+#pragma unroll
+    for (int g = 0; g < GM; ++g) Ro[g] = sel8(X, lane + g * L);
+    rr = X[1];
+    ph = X[3];
+  }
 
-  // Full derivative at the current state: owned geodesic slope + owned colours'
-  // slopes (calc_ray_derivative!, integrators_LttM.jl:9-48).
-  template <class T> __device__ __forceinline__ void full_deriv(double* dfull) {
+  // Full derivative at state R: owned geodesic slope + owned colours' slopes
+  // (calc_ray_derivative!, integrators_LttM.jl:9-48).
+  template <class T> __device__ __forceinline__ void full_deriv(const double* R, double* dfull) {
     T x[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = (T)Rf[j];
+    for (int j = 0; j < 8; ++j) x[j] = (T)R[j];
     StageOut<T> o;
     geo_rhs<KIND, T>(S, x, o);
     StageSc ss = make_stage<KIND, T>(S, o);
@@ -206,6 +214,13 @@ struct Integrator {
       if (col_ok(m)) spec_eval<KIND, T>(S, ss, scol[lane + m * L], sT[m], sg, lT[m], lI[m]);
     }
   }
+  __device__ __forceinline__ void deriv_any(bool f64, const double* R, double* d) {
+    if constexpr (MIXED) {
+      if (f64) full_deriv<double>(R, d); else full_deriv<float>(R, d);
+    } else {
+      full_deriv<double>(R, d);
+    }
+  }
 
   __device__ __forceinline__ bool all_thick() const {
     int t = 1;
@@ -216,127 +231,132 @@ struct Integrator {
   }
 
   // calc_terminate (Kerr.source/source_model.jl:168-182 and the other scenes).
-  __device__ __forceinline__ bool terminate(double max_steps, double step_count) const {
-    if (step_count >= max_steps || Rf[1] > S.r_escape || (KIND != MINK && Rf[1] <= S.r_horizon) ||
-        Rf[3] > 8 * PI)
-      return true;
+  __device__ __forceinline__ bool stop_test(double r, double phi, double max_steps, double step_count) const {
+    if (step_count >= max_steps || r > S.r_escape || (KIND != MINK && r <= S.r_horizon) || phi > 8 * PI) return true;
     return all_thick();
+  }
+  __device__ __forceinline__ bool terminate(double max_steps, double step_count) const {
+    return stop_test(rr, ph, max_steps, step_count);
   }
 
   __device__ __forceinline__ int classify() const {
     bool thick = all_thick();
-    if (!isfinite(Rf[1])) return CLS_BOUNDARY;
-    if (KIND != MINK && Rf[1] <= S.r_horizon) return CLS_HORIZON;
-    if (Rf[1] > S.r_escape) return CLS_ESCAPE;
+    if (!isfinite(rr)) return CLS_BOUNDARY;
+    if (KIND != MINK && rr <= S.r_horizon) return CLS_HORIZON;
+    if (rr > S.r_escape) return CLS_ESCAPE;
     if (thick && A.O.ncol > 0) return CLS_OPAQUE;
     return CLS_BOUNDARY;
-  }
-
-  __device__ __forceinline__ void deriv_any(bool f64, double* d) {
-    if constexpr (MIXED) {
-      if (f64) full_deriv<double>(d); else full_deriv<float>(d);
-    } else {
-      full_deriv<double>(d);
-    }
   }
 
   // Start a ray: initial state, h = -pad_max_dt, one derivative (LttM.jl:24-40, 70-71).
   __device__ __forceinline__ void start(long long ray) {
     long long pose = ray / A.npix;
     long long pix = ray - pose * A.npix;
+    double R[8];
     if (A.init_rays) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) Rf[k] = A.init_rays[pix + k * A.npix];
+      for (int k = 0; k < 8; ++k) R[k] = A.init_rays[pix + k * A.npix];
     } else {
+      // initialize_camera (initialization.jl:213-240, 327): pixel direction, then e R2 L R
       const DevCam& C = A.cams[pose];
       int i = (int)(pix % A.x_pix), j = (int)(pix / A.x_pix);
       double w = C.rho_h * ((i + 1) - 0.5) - C.W;
       double hh = C.rho_v * ((j + 1) - 0.5) - C.H;
       double th = atan(sqrt(w * w + hh * hh));
-      double ph;
-      if (w > 0) ph = atan(hh / w);
-      else if (w < 0 && hh >= 0) ph = atan(hh / w) + PI;
-      else if (w < 0 && hh < 0) ph = atan(hh / w) - PI;
-      else if (w == 0 && hh > 0) ph = PI / 2;
-      else if (w == 0 && hh < 0) ph = -PI / 2;
-      else ph = 0;
+      double phi;
+      if (w > 0) phi = atan(hh / w);
+      else if (w < 0 && hh >= 0) phi = atan(hh / w) + PI;
+      else if (w < 0 && hh < 0) phi = atan(hh / w) - PI;
+      else if (w == 0 && hh > 0) phi = PI / 2;
+      else if (w == 0 && hh < 0) phi = -PI / 2;
+      else phi = 0;
       double st, ct, sp, cp;
       sincos(th, &st, &ct);
-      sincos(ph, &sp, &cp);
+      sincos(phi, &sp, &cp);
       double v[4] = {1.0, -st * cp, -st * sp, -ct};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) Rf[k] = C.pos[k];
+      for (int k = 0; k < 4; ++k) R[k] = C.pos[k];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         double s = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) s += C.Mc[a * 4 + b] * v[b];
-        Rf[4 + a] = s;
+        R[4 + a] = s;
       }
     }
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) { sI[m] = 0; sT[m] = 0; lI[m] = 0; lT[m] = 0; }
-    sync_own();
-    h = -pad_max_dt<KIND>(S, Rf);
+    own(R);
+    h = -pad_max_dt<KIND>(S, R);
     double d[8];
-    deriv_any(!MIXED || Rf[1] < A.O.fp64_radius, d);
+    deriv_any(!MIXED || R[1] < A.O.fp64_radius, R, d);
     rejected = false;
   }
 
-  // Euler fallback (integrators_LttM.jl:62-105).  Returns true if the ray
-  // terminated inside it (the step then counts as accepted, h unchanged).
-  __device__ __forceinline__ bool euler(double ts, bool f64) {
+  // Euler fallback (integrators_LttM.jl:62-105) on the full state R.  Returns true if
+  // the ray terminated inside it (the step then counts as accepted, h unchanged).
+  __device__ __forceinline__ bool euler(double ts, bool f64, double* R) {
     double lsf[8];
     gather(LS, lsf);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) Rf[j] -= ts * lsf[j];
+    for (int j = 0; j < 8; ++j) R[j] -= ts * lsf[j];
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) { sI[m] -= ts * lI[m]; sT[m] -= ts * lT[m]; }
-    keepinbounds<KIND>(Rf);
+    keepinbounds<KIND>(R);
     // 36 Euler substeps, then one more derivative for the RK attempt; a single
     // derivative call site keeps the code compact.
+    bool done = false;
 #pragma unroll 1
     for (int it = 0; it <= 36; ++it) {
-      sync_own();
-      const bool term = terminate(2, 1);
+      const bool term = stop_test(R[1], R[3], 2, 1);
       if (term) {
-        if (it == 36) return true;
+        if (it == 36) { done = true; break; }
         continue;
       }
       double d[8];
-      deriv_any(f64, d);
+      deriv_any(f64, R, d);
       if (it == 36) break;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) Rf[j] -= ts * d[j];
+      for (int j = 0; j < 8; ++j) R[j] -= ts * d[j];
 #pragma unroll
       for (int m = 0; m < CMAX; ++m) { sI[m] -= ts * lI[m]; sT[m] -= ts * lT[m]; }
-      keepinbounds<KIND>(Rf);
+      keepinbounds<KIND>(R);
     }
-    h = -pad_max_dt<KIND>(S, Rf);
+    own(R);
+    if (done) return true;
+    h = -pad_max_dt<KIND>(S, R);
     return false;
   }
 
   // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
   __device__ __forceinline__ bool step(bool& used64) {
     const double atol = A.O.atol, rtol = A.O.rtol;
-    const double max_dt = pad_max_dt<KIND>(S, Rf);
-    const bool f64 = !MIXED || Rf[1] < A.O.fp64_radius;
-    used64 = f64;
-    double ts;
-    if (near_singularity<KIND>(Rf, h, atol, ts)) {
-      if (euler(ts, f64)) return true;  // h unchanged, accepted
+    double max_dt;
+    bool f64;
+    {
+      double R[8];
+      gather(Ro, R);
+      max_dt = pad_max_dt<KIND>(S, R);
+      f64 = !MIXED || R[1] < A.O.fp64_radius;
+      double ts;
+      if (near_singularity<KIND>(R, h, atol, ts)) {
+        if (euler(ts, f64, R)) {
+          used64 = f64;
+          return true;  // h unchanged, accepted
+        }
+      }
     }
+    used64 = f64;
     const double hh = h;
 
     // ---- geodesic stages 2..7 ------------------------------------------------
-    double acc[GM][6], eg[GM], ns[GM];
+    double acc[GM][6], eg[GM], ns[GM], yo[GM];
 #pragma unroll
     for (int g = 0; g < GM; ++g) {
 #pragma unroll
       for (int t = 0; t < 6; ++t) acc[g][t] = k1c(t) * LS[g];
       eg[g] = DPE[1] * LS[g];
     }
-    double X7[8];
     int vac_all = 1;
 #pragma unroll 1
     for (int s = 2; s <= 7; ++s) {
@@ -344,7 +364,7 @@ struct Integrator {
 #pragma unroll
       for (int g = 0; g < GM; ++g) {
         xo[g] = Ro[g] + hh * acc[g][0];
-        if (s == 7) ns[g] = acc[g][0];
+        if (s == 7) { ns[g] = acc[g][0]; yo[g] = xo[g]; }
       }
       double X[8];
       gather(xo, X);
@@ -378,20 +398,15 @@ struct Integrator {
         for (int t = 0; t < 5; ++t) acc[g][t] = acc[g][t + 1];
         acc[g][5] = 0;
       }
-      if (s == 7) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) X7[j] = X[j];
-      }
     }
     __builtin_amdgcn_wave_barrier();
 
-    // error of the owned geodesic components; y = X7 (same expression)
+    // error of the owned geodesic components; y = stage-7 point (same expression)
     double e2 = 0;
 #pragma unroll
     for (int g = 0; g < GM; ++g) {
-      double y = sel8(X7, lane + g * L);
-      double dl = y - Ro[g] - hh * eg[g];
-      double tol = atol + fmax(fabs(Ro[g]), fabs(y)) * rtol;
+      double dl = yo[g] - Ro[g] - hh * eg[g];
+      double tol = atol + fmax(fabs(Ro[g]), fabs(yo[g])) * rtol;
       double q = dl / tol;
       e2 += q * q;
     }
@@ -459,17 +474,17 @@ struct Integrator {
     } else if (err > 1) {
       accept = false;
       if (rejected && fabs(nh) >= fabs(hh)) nh = hh / 2;
-      keepinbounds<KIND>(Rf);
-      sync_own();
     } else {
       accept = true;
       if (fabs(nh) > 5 * fabs(hh)) nh = 5 * hh;
     }
+    {
+      double Y[8];
+      gather(accept ? yo : Ro, Y);
+      keepinbounds<KIND>(Y);
+      own(Y);
+    }
     if (accept) {
-      keepinbounds<KIND>(X7);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) Rf[j] = X7[j];
-      sync_own();
 #pragma unroll
       for (int g = 0; g < GM; ++g) LS[g] = ns[g];
 #pragma unroll
@@ -522,8 +537,10 @@ struct Integrator {
   }
 };
 
+
 template <int KIND, int L, int CMAX, bool MIXED>
-__global__ void __launch_bounds__(BLOCK) rk45_kernel(const KArgs* __restrict__ Ap) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_WAVES_PER_EU))) rk45_kernel(
+    const KArgs* __restrict__ Ap) {
   const KArgs& A = *Ap;
   extern __shared__ double smem[];
   const int ncol = A.O.ncol;

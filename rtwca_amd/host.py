@@ -18,7 +18,7 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librtw.so")
+LIB_PATH = os.environ.get("RTW_LIBRARY") or os.path.join(HERE, "librtw.so")
 GAUNT_PATH = os.path.join(HERE, "data", "gauntff_merged_Z01.dat")
 
 _lib = None

@@ -1,0 +1,66 @@
+"""Generate the committed fixtures under tests/golden/ from the oracle.
+
+    python tests/golden/make_golden.py
+
+* oracle_<cfg>_<npix>px_<nlam>.npz: small renders of the benchmark workloads
+  (XYZ, class, attempted/accepted steps, final states) -- the golden vectors that the
+  GPU parity tests compare against and that the CPU tests use to detect any change
+  in the oracle;
+* flops_per_step.json: arithmetic work of one attempted RK45 step per workload,
+  measured by running the oracle on a counting scalar (add/sub/mul/div/sqrt = 1,
+  each transcendental call = 1; the oracle does not fuse multiply-adds).
+
+The reference ships no golden vectors for this path (it has no tests), so these
+fixtures are the oracle's own outputs; the oracle is pinned independently by
+tests/test_oracle.py.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle as O  # noqa: E402
+from rtwca_amd import scenes  # noqa: E402
+
+FIXTURES = [(1, 16, 64), (2, 16, 16), (3, 16, 64), (3, 12, 8), (4, 12, 16)]
+
+
+def fixture_name(idx, x_pix, nlam):
+    return os.path.join(HERE, "oracle_cfg%d_%dpx_%dl.npz" % (idx, x_pix, nlam))
+
+
+def make_fixtures():
+    for idx, x_pix, nlam in FIXTURES:
+        w = scenes.config(idx, x_pix=x_pix, n_lambda=nlam)
+        r = O.render(w.scene, w.cameras[0], w.integ, w.lam, nthreads=os.cpu_count() or 1, want_rays=True)
+        np.savez_compressed(fixture_name(idx, x_pix, nlam), xyz=r["xyz"], cls=r["cls"], steps=r["steps"],
+                            rays=r["rays"], lam=np.array(w.lam))
+        print("wrote", fixture_name(idx, x_pix, nlam), "steps", int(r["steps"][:, 0].sum()))
+
+
+def make_flops():
+    out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
+                         "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
+    for idx, x_pix in ((1, 64), (2, 64), (3, 64), (4, 64)):
+        w = scenes.config(idx, x_pix=x_pix)
+        npix = x_pix * x_pix
+        sample = list(range(0, npix, max(1, npix // 96)))
+        c = O.count_ops(w.scene, w.cameras[0], w.integ, w.lam, sample)
+        out["cfg%d" % idx] = {"workload": w.name, "arith_per_step": c["arith"] / c["steps"],
+                              "trans_per_step": c["trans"] / c["steps"],
+                              "flops_per_step": (c["arith"] + c["trans"]) / c["steps"],
+                              "sample_rays": len(sample), "sample_steps": int(c["steps"])}
+        print(idx, out["cfg%d" % idx])
+    with open(os.path.join(HERE, "flops_per_step.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    make_fixtures()
+    make_flops()
