@@ -4,10 +4,12 @@
 
 One "step" of this benchmark is one full render of the workload (every ray of the
 1024x1024 camera integrated to termination with 64 wavelengths) by the
-mixed-precision persistent kernel, camera rays generated on the device and the
-per-pixel XYZ left in HBM.  Timed with HIP events on the launch stream.  Besides
-the headline line it renders once with the plain FP64 kernel for the speed-up and
-the per-pixel XYZ error of the mixed-precision image.
+mixed-precision kernels on the wavefront schedule (persistent passes fed by
+device queues), camera rays generated on the device and the per-pixel XYZ left in
+HBM.  Timed with HIP events on the launch stream.  Besides the headline line it
+renders once with the plain FP64 kernel (one lane group per ray, grid launch) and
+once with the FP64 kernels on the wavefront schedule, for the speed-ups and the
+per-pixel XYZ error of the mixed-precision image against FP64.
 
 cpu_baseline: the oracle (CPU FP64 restatement of the reference integrator) on a
 fixed pixel sample, on the host's cores, threaded like LttM.jl:156-165.
@@ -37,7 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
-    ap.add_argument("--schedule", type=int, default=1)
+    ap.add_argument("--schedule", type=int, default=2)
     return ap.parse_args()
 
 
@@ -46,12 +48,13 @@ def flops_per_step():
         return json.load(f)["cfg3"]["flops_per_step"]
 
 
-def gpu_render(ctx, w, precision, steps, warmup, torch):
+def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
     """Timed renders through rtw_render_dev; returns (ms per render list, xyz, cls, steps tensor)."""
     from rtwca_amd import abi, host
     cams = (abi.Camera * len(w.cameras))(*w.cameras)
     integ = abi.make_integrator(w.integ.rtol, w.integ.atol, w.integ.max_dt_scale, w.integ.max_steps,
-                                precision, w.integ.schedule, w.integ.fp64_radius)
+                                precision, w.integ.schedule if schedule is None else schedule,
+                                w.integ.fp64_radius)
     lam = np.ascontiguousarray(w.lam, dtype=np.float64)
     c0 = w.cameras[0]
     ny = int(np.rint(c0.x_pix * np.tan(np.pi * c0.fov_vert / 360) / np.tan(np.pi * c0.fov_hor / 360)))
@@ -127,7 +130,7 @@ def main():
         "dtype": "f32 stages + f64 state/controller", "data": "synthetic",
         "config": {"workload": w.name, "rays": npix, "n_lambda": len(w.lam), "rtol": w.integ.rtol,
                    "atol": w.integ.atol, "max_dt_scale": w.integ.max_dt_scale, "max_steps": w.integ.max_steps,
-                   "precision": "mixed", "schedule": "persistent" if a.schedule else "grid"},
+                   "precision": "mixed", "schedule": {0: "grid", 1: "persistent", 2: "wavefront"}[a.schedule]},
         "rk45_steps_per_s": att / (ms * 1e-3),
         "steps_attempted": att,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU, "unit": "TFLOP/s",
@@ -135,19 +138,24 @@ def main():
                      "flops_per_step": F},
     }
     if not a.no_fp64:
-        t64, xyz64, cls64, stp64 = gpu_render(ctx, w, abi.PREC_FP64, 1, 0, torch)
-        ms64 = float(np.mean(t64))
-        att64 = int(stp64[:, 0].to(torch.int64).sum().item())
-        X, X64 = xyz.cpu().numpy(), xyz64.cpu().numpy()
-        ref = np.abs(X64).max(axis=1)
-        m = ref > 0
-        rel = np.abs(X - X64).max(axis=1)[m] / ref[m]
-        out["fp64"] = {"rays_per_s": npix / (ms64 * 1e-3), "ms": ms64, "rk45_steps_per_s": att64 / (ms64 * 1e-3),
-                       "achieved_tflops": att64 * F / (ms64 * 1e-3) / 1e12}
-        out["speedup_vs_fp64"] = ms64 / ms
-        out["xyz_rel_err_median"] = float(np.median(rel)) if m.any() else 0.0
-        out["xyz_rel_err_p99"] = float(np.percentile(rel, 99)) if m.any() else 0.0
-        out["class_agreement"] = float((cls.cpu().numpy() == cls64.cpu().numpy()).mean())
+        X = xyz.cpu().numpy()
+        for tag, sched in (("fp64_plain", abi.SCHED_GRID), ("fp64_wavefront", abi.SCHED_WAVEFRONT)):
+            t64, xyz64, cls64, stp64 = gpu_render(ctx, w, abi.PREC_FP64, 1, 0, torch, schedule=sched)
+            ms64 = float(np.mean(t64))
+            att64 = int(stp64[:, 0].to(torch.int64).sum().item())
+            X64 = xyz64.cpu().numpy()
+            ref = np.abs(X64).max(axis=1)
+            m = ref > 0
+            rel = np.abs(X - X64).max(axis=1)[m] / ref[m]
+            out[tag] = {"rays_per_s": npix / (ms64 * 1e-3), "ms": ms64, "rk45_steps_per_s": att64 / (ms64 * 1e-3),
+                        "achieved_tflops": att64 * F / (ms64 * 1e-3) / 1e12,
+                        "speedup_of_mixed": ms64 / ms,
+                        "mixed_xyz_rel_err_median": float(np.median(rel)) if m.any() else 0.0,
+                        "mixed_xyz_rel_err_p99": float(np.percentile(rel, 99)) if m.any() else 0.0,
+                        "class_agreement": float((cls.cpu().numpy() == cls64.cpu().numpy()).mean())}
+        out["speedup_vs_fp64_plain"] = out["fp64_plain"]["speedup_of_mixed"]
+        out["xyz_rel_err_median"] = out["fp64_plain"]["mixed_xyz_rel_err_median"]
+        out["class_agreement"] = out["fp64_plain"]["class_agreement"]
     if not a.no_cpu:
         cb = cpu_baseline(scenes.config(3, x_pix=a.x_pix), a.cpu_sample)
         out["cpu_baseline"] = cb

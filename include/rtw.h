@@ -96,7 +96,9 @@ enum rtw_precision {
 };
 enum rtw_schedule {
   RTW_SCHED_GRID = 0,       /* one launch slot per ray */
-  RTW_SCHED_PERSISTENT = 1  /* persistent lane groups pulling rays from a device work queue */
+  RTW_SCHED_PERSISTENT = 1, /* persistent lane groups pulling rays from a device work queue */
+  RTW_SCHED_WAVEFRONT = 2   /* persistent passes: lane groups while the transfer equations are live,
+                               one ray per lane through frozen vacuum stretches (see rtw_render) */
 };
 
 /* The integrator arguments of gen_image (main/Iterators/LttM/LttM.jl:104-107),
@@ -130,6 +132,7 @@ typedef struct rtw_stats {
   int64_t steps_attempted;  /* sum over rays of attempted RK45 steps (LttM.jl:43-46) */
   int64_t steps_accepted;
   int64_t steps_fp64;       /* MIXED: attempted steps that ran in FP64 */
+  int64_t steps_transfer;   /* attempted steps with matter at some stage or nonzero transfer slopes */
   int32_t lanes_per_ray;
   int32_t reserved;
 } rtw_stats;

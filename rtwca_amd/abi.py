@@ -26,6 +26,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 SCHED_GRID = 0
 SCHED_PERSISTENT = 1
+SCHED_WAVEFRONT = 2
 
 CLASS_HORIZON = 1
 CLASS_ESCAPE = 2
@@ -67,7 +68,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_int64),
         ("steps_attempted", C.c_int64), ("steps_accepted", C.c_int64),
-        ("steps_fp64", C.c_int64), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
+        ("steps_fp64", C.c_int64), ("steps_transfer", C.c_int64), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

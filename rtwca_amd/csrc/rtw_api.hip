@@ -42,7 +42,7 @@ struct rtw_ctx {
   std::string err;
   bool gaunt_loaded = false;
   std::vector<float> gaunt;  // 146 x 81 row-major
-  DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs;
+  DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs, pool;
 };
 
 static thread_local std::string g_create_err;
@@ -478,15 +478,73 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   A.out_cls = d_cls;
   A.out_steps = d_steps;
   A.out_rays = d_rays;
-  A.queue = c->queue.as<unsigned>();
   A.counters = c->counters.as<unsigned long long>();
-  A.persistent = opts->schedule == RTW_SCHED_PERSISTENT ? 1 : 0;
+  A.persistent = opts->schedule != RTW_SCHED_GRID ? 1 : 0;
+  A.wavefront = opts->schedule == RTW_SCHED_WAVEFRONT ? 1 : 0;
+  const bool mixed = opts->precision == RTW_PREC_MIXED;
+
+  unsigned* qc = c->queue.as<unsigned>();  // [0] head, [1] output count
+  unsigned* qbuf[2] = {nullptr, nullptr};
+  if (A.wavefront) {
+    // ray pool + two ping-pong queues, carved from one allocation
+    const size_t N = (size_t)total, nc = (size_t)n_lambda;
+    const size_t nd = N * (8 + 8 + 2 * nc + 1);
+    const size_t bytes = nd * sizeof(double) + 2 * N * sizeof(int32_t) + 2 * N * sizeof(unsigned) + N + 256;
+    HIPCHK(c, c->pool.ensure(bytes));
+    char* b = c->pool.as<char>();
+    double* d = reinterpret_cast<double*>(b);
+    A.pool.geo = d; d += 8 * N;
+    A.pool.ls = d; d += 8 * N;
+    A.pool.sI = d; d += nc * N;
+    A.pool.sT = d; d += nc * N;
+    A.pool.h = d; d += N;
+    int32_t* ip = reinterpret_cast<int32_t*>(d);
+    A.pool.att = ip; ip += N;
+    A.pool.acc = ip; ip += N;
+    unsigned* up = reinterpret_cast<unsigned*>(ip);
+    qbuf[0] = up; up += N;
+    qbuf[1] = up; up += N;
+    A.pool.flags = reinterpret_cast<uint8_t*>(up);
+  }
 
   HIPCHK(c, c->kargs.ensure(sizeof(KArgs)));
   HIPCHK(c, hipMemcpyAsync(c->kargs.p, &A, sizeof(KArgs), hipMemcpyHostToDevice, st));
+  const KArgs* dA = c->kargs.as<KArgs>();
   LaunchInfo li;
   if (timed) HIPCHK(c, hipEventRecord(c->ev0, st));
-  hipError_t e = launch_rk45(A, c->kargs.as<KArgs>(), opts->precision == RTW_PREC_MIXED, opts->lanes_per_ray, c->num_cu, st, li);
+  hipError_t e = hipSuccess;
+  if (!A.wavefront) {
+    PassArgs Q = {nullptr, (unsigned)total, qc, nullptr, nullptr, 1};
+    e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
+  } else {
+    // Wavefront schedule: the lane-group kernel integrates camera rays until they
+    // terminate or their transfer state freezes in vacuum; frozen rays continue in
+    // the ray-per-lane vacuum kernel until they terminate or meet matter / a pole,
+    // and go back; repeat until both queues are empty.  Every pass is a persistent
+    // launch fed by a device queue; the host only reads the queue lengths.
+    PassArgs Q = {nullptr, (unsigned)total, qc, qbuf[0], qc + 1, 1};
+    fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
+    e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
+    int which = 1;  // next pass kind
+    unsigned nin = 0;
+    int cur = 0;    // qbuf[cur] holds the queue produced by the last pass
+    const long long max_passes = 2 * (long long)opts->max_steps + 16;
+    for (long long pass = 0; e == hipSuccess && pass < max_passes; ++pass) {
+      HIPCHK(c, hipMemcpyAsync(&nin, qc + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      if (nin == 0) break;
+      PassArgs P = {qbuf[cur], nin, qc, qbuf[1 - cur], qc + 1, 0};
+      fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
+      LaunchInfo lp;
+      e = launch_pass(which, A, dA, P, mixed, c->num_cu, st, lp);
+      cur = 1 - cur;
+      which = 1 - which;
+    }
+    if (e == hipSuccess) {
+      finish_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(dA);
+      e = hipGetLastError();
+    }
+  }
   if (e != hipSuccess) return fail(c, RTW_E_DEVICE, std::string("integrator launch: ") + hipGetErrorString(e) + " " + li.msg);
   if (timed) HIPCHK(c, hipEventRecord(c->ev1, st));
   if (stats) {
@@ -499,11 +557,12 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
 
 static int collect_stats(rtw_ctx* c, rtw_stats* stats, bool timed) {
   if (!stats) return RTW_OK;
-  unsigned long long cnt[3] = {0, 0, 0};
+  unsigned long long cnt[4] = {0, 0, 0, 0};
   HIPCHK(c, hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
   stats->steps_attempted = (int64_t)cnt[0];
   stats->steps_accepted = (int64_t)cnt[1];
   stats->steps_fp64 = (int64_t)cnt[2];
+  stats->steps_transfer = (int64_t)cnt[3];
   if (timed) {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));

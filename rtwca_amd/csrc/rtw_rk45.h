@@ -59,13 +59,27 @@ __device__ __forceinline__ double k1c(int t) {
   }
 }
 
+// Ray pool of the wavefront schedule: the state of every ray between passes,
+// structure-of-arrays over the N rays of the render (component-major).
+enum { RAY_REJECTED = 1, RAY_DONE = 2 };
+struct RayPool {
+  double* geo;    // [8][N]  t r theta phi k^t k^r k^theta k^phi
+  double* ls;     // [8][N]  geodesic part of the last slope
+  double* sI;     // [ncol][N]  I/nu^3
+  double* sT;     // [ncol][N]  tau
+  double* h;      // [N]
+  int32_t* att;   // [N] attempted steps so far
+  int32_t* acc;   // [N] accepted steps so far
+  uint8_t* flags; // [N] RAY_*
+};
+
 struct KArgs {
   DevScene S;
   DevOpts O;
   const DevCam* cams;
   int x_pix, y_pix;
   long long npix;   // pixels per pose
-  long long total;  // rays in the launch
+  long long total;  // rays in the render
   const double* init_rays;  // optional (npix, 8) column-major, single pose
   const DevColor* colors;
   const float2* gcols;      // 146 rows of the two Gaunt columns
@@ -73,10 +87,25 @@ struct KArgs {
   uint8_t* out_cls;
   int32_t* out_steps;
   double* out_rays;         // optional (total, n) column-major
-  unsigned* queue;
-  unsigned long long* counters;  // attempted, accepted, fp64
+  unsigned long long* counters;  // attempted, accepted, fp64, transfer steps
   int persistent;
+  int wavefront;            // rays hand frozen-vacuum stretches to vacuum_kernel
+  RayPool pool;
 };
+
+// Per-launch queue arguments (by value).  A pass takes rays from qin[0..nin)
+// (or ray ids 0..nin when camera != 0, generating camera rays) through the
+// atomic head counter, and appends rays it hands over to qout / *nout.
+struct PassArgs {
+  const unsigned* qin;
+  unsigned nin;
+  unsigned* head;
+  unsigned* qout;
+  unsigned* nout;
+  int camera;
+};
+
+constexpr double FREEZE_REL = 1.0 / 72057594037927936.0;  // 2^-56
 
 // v[j] for a lane-dependent j without dynamic register indexing: an OR of masked
 // bit patterns (a select chain would be folded back into an indexed load, which
@@ -177,6 +206,8 @@ struct Integrator {
   double sI[CMAX], sT[CMAX], lI[CMAX], lT[CMAX];
   double h;
   bool rejected;
+  bool last_vac = false;              // last step accepted with every stage in vacuum
+  unsigned long long n_transfer = 0;  // steps that ran the full transfer stages
 
   __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, StageSc* st, double* cd)
       : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sst(st), cand(cd) {}
@@ -427,6 +458,7 @@ struct Integrator {
         CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
       }
     } else {
+      n_transfer++;
 #pragma unroll
       for (int m = 0; m < CMAX; ++m) {
         if (!col_ok(m)) continue;
@@ -495,7 +527,82 @@ struct Integrator {
     }
     h = nh;
     rejected = !accept;
+    last_vac = accept && vac_all;
     return accept;
+  }
+
+  // ---- wavefront schedule: hand-over through the ray pool ---------------------
+  // Camera rays whose initial transfer slopes are all exactly zero (camera in vacuum).
+  __device__ __forceinline__ bool frozen_now() const {
+    int z = 1;
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m)
+      if (col_ok(m) && (lI[m] != 0.0 || lT[m] != 0.0)) z = 0;
+    return gand<L>(z) != 0;
+  }
+  // After an accepted all-vacuum step: freeze when every last slope is below
+  // 2^-56 of its component (h * b1 * slope can no longer change the state), and
+  // then drop the slopes.
+  __device__ __forceinline__ bool try_freeze() {
+    int z = 1;
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) {
+      if (!col_ok(m)) continue;
+      if (!(fabs(lI[m]) <= FREEZE_REL * fabs(sI[m])) || !(fabs(lT[m]) <= FREEZE_REL * fabs(sT[m]))) z = 0;
+    }
+    if (!gand<L>(z)) return false;
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) { lI[m] = 0; lT[m] = 0; }
+    return true;
+  }
+  __device__ __forceinline__ void store(long long ray, int att, int acc, int flags) {
+    const RayPool& P = A.pool;
+    const long long N = A.total;
+#pragma unroll
+    for (int g = 0; g < GM; ++g) {
+      P.geo[ray + (long long)(lane + g * L) * N] = Ro[g];
+      P.ls[ray + (long long)(lane + g * L) * N] = LS[g];
+    }
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) {
+      if (!col_ok(m)) continue;
+      long long c = lane + m * L;
+      P.sI[ray + c * N] = sI[m];
+      P.sT[ray + c * N] = sT[m];
+    }
+    if (lane == 0) {
+      P.h[ray] = h;
+      P.att[ray] = att;
+      P.acc[ray] = acc;
+      P.flags[ray] = (uint8_t)(flags | (rejected ? RAY_REJECTED : 0));
+    }
+  }
+  // Resume a ray handed back by the vacuum kernel: its transfer slopes are zero.
+  __device__ __forceinline__ void resume(long long ray, int& att, int& acc) {
+    const RayPool& P = A.pool;
+    const long long N = A.total;
+#pragma unroll
+    for (int g = 0; g < GM; ++g) {
+      Ro[g] = P.geo[ray + (long long)(lane + g * L) * N];
+      LS[g] = P.ls[ray + (long long)(lane + g * L) * N];
+    }
+    double R[8];
+    gather(Ro, R);
+    rr = R[1];
+    ph = R[3];
+#pragma unroll
+    for (int m = 0; m < CMAX; ++m) {
+      lI[m] = 0;
+      lT[m] = 0;
+      if (!col_ok(m)) { sI[m] = 0; sT[m] = 0; continue; }
+      long long c = lane + m * L;
+      sI[m] = P.sI[ray + c * N];
+      sT[m] = P.sT[ray + c * N];
+    }
+    h = P.h[ray];
+    rejected = (P.flags[ray] & RAY_REJECTED) != 0;
+    att = P.att[ray];
+    acc = P.acc[ray];
   }
 
   // calc_xyY sums (color_theory.jl:67-102) and the outputs of one ray.
@@ -540,7 +647,7 @@ struct Integrator {
 
 template <int KIND, int L, int CMAX, bool MIXED>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_WAVES_PER_EU))) rk45_kernel(
-    const KArgs* __restrict__ Ap) {
+    const KArgs* __restrict__ Ap, const PassArgs Q) {
   const KArgs& A = *Ap;
   extern __shared__ double smem[];
   const int ncol = A.O.ncol;
@@ -567,29 +674,47 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   const double max_steps = A.O.max_steps;
   while (true) {
     if (need) {
+      long long q;
       if (A.persistent) {
         unsigned v = 0;
-        if (lane == 0) v = atomicAdd(A.queue, 1u);
-        ray = (long long)(unsigned)__shfl((int)v, 0, L);
+        if (lane == 0) v = atomicAdd(Q.head, 1u);
+        q = (long long)(unsigned)__shfl((int)v, 0, L);
       } else {
-        ray = next_static;
+        q = next_static;
         next_static += groups;
       }
-      if (ray >= A.total) break;
-      it.start(ray);
-      attempted = 0;
-      accepted = 0;
+      if (q >= (long long)Q.nin) break;
       need = false;
+      if (Q.camera) {
+        ray = q;
+        it.start(ray);
+        attempted = 0;
+        accepted = 0;
+        if (A.wavefront && it.frozen_now()) {  // starts in vacuum: straight to the vacuum kernel
+          it.store(ray, 0, 0, 0);
+          if (lane == 0) Q.qout[atomicAdd(Q.nout, 1u)] = (unsigned)ray;
+          need = true;
+          continue;
+        }
+      } else {
+        ray = Q.qin[q];
+        it.resume(ray, attempted, accepted);
+      }
     }
     bool used64;
     bool acc = it.step(used64);
     attempted++;
     accepted += acc ? 1 : 0;
+    tot_att++;
+    tot_acc += acc ? 1 : 0;
     tot_64 += used64 ? 1 : 0;
     if (it.terminate(max_steps, (double)attempted)) {
-      it.finish(ray, attempted, accepted);
-      tot_att += attempted;
-      tot_acc += accepted;
+      if (A.wavefront) it.store(ray, attempted, accepted, RAY_DONE);
+      else it.finish(ray, attempted, accepted);
+      need = true;
+    } else if (A.wavefront && it.last_vac && it.try_freeze()) {
+      it.store(ray, attempted, accepted, 0);
+      if (lane == 0) Q.qout[atomicAdd(Q.nout, 1u)] = (unsigned)ray;
       need = true;
     }
   }
@@ -597,6 +722,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
     atomicAdd(&A.counters[0], tot_att);
     atomicAdd(&A.counters[1], tot_acc);
     atomicAdd(&A.counters[2], tot_64);
+    atomicAdd(&A.counters[3], it.n_transfer);
   }
 }
 

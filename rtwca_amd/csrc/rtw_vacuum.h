@@ -1,0 +1,216 @@
+// rtw_vacuum.h -- ray-per-lane RK45 kernel for rays whose transfer state is frozen.
+//
+// In vacuum the transfer slopes vanish at every stage and only the averaged
+// last slope k1 carries the (I, tau) components forward; it shrinks by
+// b1 = 35/384 per accepted step (integrators_LttM.jl:147-150, 163-165).  Once
+// |k1| < 2^-56 |state| for every colour, adding it can no longer change the FP64
+// state and its error-norm terms are ~1e-27 of any other term, so the transfer
+// components are frozen: a step is then a geodesic-only DP5 step whose error
+// norm still divides by the full n = 8 + 2N (integrators_LttM.jl:185).  Such
+// steps run here, one ray per lane (64 rays per wave), instead of in the
+// lane-group kernel where every lane of a ray's group repeats the geodesic.
+//
+// A ray goes back to the group kernel (without taking the step) when a stage
+// point of the attempted step lies in matter or the pole fallback
+// (near_singularity) triggers; the group kernel redoes that attempt with the
+// full transfer equations.
+#pragma once
+#include "rtw_device.h"
+#include "rtw_rk45.h"
+
+namespace rtw {
+
+#ifndef RTW_VAC_WAVES_PER_EU
+#define RTW_VAC_WAVES_PER_EU 2
+#endif
+
+template <int KIND, bool MIXED>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_VAC_WAVES_PER_EU)))
+vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
+  const KArgs& A = *Ap;
+  const DevScene& S = A.S;
+  const RayPool& P = A.pool;
+  const long long N = A.total;
+  const double atol = A.O.atol, rtol = A.O.rtol, max_steps = A.O.max_steps;
+  const double inv_n = 1.0 / (double)A.O.n;
+  unsigned long long tot_att = 0, tot_acc = 0, tot_64 = 0;
+
+  long long ray = 0;
+  double R[8], LS[8], h = 0;
+  bool rej = false;
+  int att = 0, acc = 0;
+  bool need = true;
+  while (true) {
+    if (need) {
+      unsigned q = atomicAdd(Q.head, 1u);
+      if (q >= Q.nin) break;
+      ray = Q.qin[q];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        R[j] = P.geo[ray + j * N];
+        LS[j] = P.ls[ray + j * N];
+      }
+      h = P.h[ray];
+      rej = (P.flags[ray] & RAY_REJECTED) != 0;
+      att = P.att[ray];
+      acc = P.acc[ray];
+      need = false;
+    }
+    const double max_dt = pad_max_dt<KIND>(S, R);
+    double ts;
+    bool handoff = near_singularity<KIND>(R, h, atol, ts);
+    const bool f64 = !MIXED || R[1] < A.O.fp64_radius;
+    double Y[8], NS[8], E[8];
+    if (!handoff) {
+      // stages 2..7, fully unrolled (k_s kept in registers)
+      double K[7][8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) K[0][j] = LS[j];
+#pragma unroll
+      for (int s = 2; s <= 7; ++s) {
+        double X[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          double a = 0;
+          // forward sum in the reference's order: k1*a_s1 + k2*a_s2 + ...
+#pragma unroll
+          for (int q = 1; q < s; ++q) {
+            double c = (q == 1) ? k1c(s - 2) : DPC[q][s - q];
+            a = (q == 1) ? c * K[0][j] : a + c * K[q - 1][j];
+          }
+          X[j] = R[j] + h * a;
+          if (s == 7) { NS[j] = a; Y[j] = X[j]; }
+        }
+        bool m;
+        if (!MIXED || f64) {
+          StageOut<double> o;
+          geo_rhs<KIND, double>(S, X, o);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) K[s - 1][j] = o.d[j];
+          m = o.matter;
+        } else {
+          float xf[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) xf[j] = (float)X[j];
+          StageOut<float> o;
+          geo_rhs<KIND, float>(S, xf, o);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) K[s - 1][j] = (double)o.d[j];
+          m = o.matter;
+        }
+        handoff |= m;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        double e = DPE[1] * K[0][j];
+#pragma unroll
+        for (int s = 3; s <= 7; ++s) e += DPE[s] * K[s - 1][j];
+        E[j] = e;
+      }
+    }
+    if (handoff) {
+      // back to the group kernel, this attempt not taken
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        P.geo[ray + j * N] = R[j];
+        P.ls[ray + j * N] = LS[j];
+      }
+      P.h[ray] = h;
+      P.flags[ray] = rej ? RAY_REJECTED : 0;
+      P.att[ray] = att;
+      P.acc[ray] = acc;
+      unsigned slot = atomicAdd(Q.nout, 1u);
+      Q.qout[slot] = (unsigned)ray;
+      need = true;
+      continue;
+    }
+    // error norm (the frozen transfer components contribute exactly 0) and controller
+    double e2 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      double dl = Y[j] - R[j] - h * E[j];
+      double q = dl / (atol + fmax(fabs(R[j]), fabs(Y[j])) * rtol);
+      e2 += q * q;
+    }
+    double err = sqrt(e2 * inv_n);
+    double nh = 0.9 * h * pow(1.0 / err, 0.2);
+    if (fabs(nh) > max_dt) nh = -max_dt;
+    bool accept;
+    if (err == 0 || !isfinite(nh)) {
+      accept = true;
+      nh = h;
+    } else if (err > 1) {
+      accept = false;
+      if (rej && fabs(nh) >= fabs(h)) nh = h / 2;
+    } else {
+      accept = true;
+      if (fabs(nh) > 5 * fabs(h)) nh = 5 * h;
+    }
+    att++;
+    tot_att++;
+    tot_64 += f64 ? 1 : 0;
+    if (accept) {
+      keepinbounds<KIND>(Y);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) { R[j] = Y[j]; LS[j] = NS[j]; }
+      acc++;
+      tot_acc++;
+    } else {
+      keepinbounds<KIND>(R);
+    }
+    h = nh;
+    rej = !accept;
+    // calc_terminate without the optical-depth test (frozen tau was not thick)
+    if ((double)att >= max_steps || R[1] > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) || R[3] > 8 * PI) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) P.geo[ray + j * N] = R[j];
+      P.h[ray] = h;
+      P.att[ray] = att;
+      P.acc[ray] = acc;
+      P.flags[ray] = RAY_DONE;
+      need = true;
+    }
+  }
+  if (A.counters) {
+    atomicAdd(&A.counters[0], tot_att);
+    atomicAdd(&A.counters[1], tot_acc);
+    atomicAdd(&A.counters[2], tot_64);
+  }
+}
+
+// XYZ (calc_xyY sums, color_theory.jl:67-102), class and step counts of every ray
+// of a wavefront render, from the ray pool.  One thread per ray.
+static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restrict__ Ap) {
+  const KArgs& A = *Ap;
+  const RayPool& P = A.pool;
+  const long long N = A.total;
+  long long ray = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (ray >= N) return;
+  double X = 0, Y = 0, Z = 0;
+  bool thick = A.O.ncol > 0;
+  for (int c = 0; c < A.O.ncol; ++c) {
+    const DevColor& k = A.colors[c];
+    double I = P.sI[ray + c * N] * k.cx;
+    X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
+    if (!(P.sT[ray + c * N] >= A.O.tau_thr)) thick = false;
+  }
+  double r = P.geo[ray + N];
+  int cls;
+  if (!isfinite(r)) cls = CLS_BOUNDARY;
+  else if (A.S.kind != MINK && r <= A.S.r_horizon) cls = CLS_HORIZON;
+  else if (r > A.S.r_escape) cls = CLS_ESCAPE;
+  else if (thick) cls = CLS_OPAQUE;
+  else cls = CLS_BOUNDARY;
+  if (A.out_xyz) { A.out_xyz[3 * ray] = X; A.out_xyz[3 * ray + 1] = Y; A.out_xyz[3 * ray + 2] = Z; }
+  if (A.out_cls) A.out_cls[ray] = (uint8_t)cls;
+  if (A.out_steps) { A.out_steps[2 * ray] = P.att[ray]; A.out_steps[2 * ray + 1] = P.acc[ray]; }
+  if (A.out_rays) {
+    for (int j = 0; j < 8; ++j) A.out_rays[ray + j * N] = P.geo[ray + j * N];
+    for (int c = 0; c < A.O.ncol; ++c) {
+      A.out_rays[ray + (8 + 2 * c) * N] = P.sI[ray + c * N];
+      A.out_rays[ray + (9 + 2 * c) * N] = P.sT[ray + c * N];
+    }
+  }
+}
+
+}  // namespace rtw

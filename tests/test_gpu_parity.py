@@ -70,6 +70,27 @@ def test_mixed_precision_vs_fp64(ctx, idx, x_pix, nlam):
     assert gm["stats"]["steps_fp64"] <= gm["stats"]["steps_attempted"]
 
 
+@pytest.mark.parametrize("idx,x_pix,nlam", FIXTURES)
+def test_wavefront_fp64_matches_golden(ctx, idx, x_pix, nlam):
+    gold = np.load(os.path.join(GOLD, "oracle_cfg%d_%dpx_%dl.npz" % (idx, x_pix, nlam)))
+    w = scenes.config(idx, x_pix=x_pix, n_lambda=nlam, schedule=abi.SCHED_WAVEFRONT)
+    g = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    _check(g, gold["xyz"], gold["cls"], 1e-5, cls_bar=0.99)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_wavefront_matches_single_kernel(ctx, prec):
+    """Freezing the transfer state in vacuum changes nothing beyond the last bits."""
+    w = scenes.config(3, x_pix=48, n_lambda=64, precision=prec, schedule=abi.SCHED_PERSISTENT)
+    a = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    w.integ.schedule = abi.SCHED_WAVEFRONT
+    b = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    assert (a["cls"] == b["cls"]).mean() >= 0.999
+    rel, m = _rel(b["xyz"], a["xyz"])
+    assert np.median(rel) < 1e-12
+    assert b["stats"]["steps_attempted"] == pytest.approx(a["stats"]["steps_attempted"], rel=1e-3)
+
+
 def test_schedules_bit_identical(ctx):
     w = scenes.config(3, x_pix=24, n_lambda=16)
     w.integ.schedule = abi.SCHED_GRID
@@ -148,7 +169,7 @@ def test_argument_errors(ctx):
 
 
 def test_full_size_properties(ctx):
-    w = scenes.config(3, x_pix=1024, precision=abi.PREC_MIXED)
+    w = scenes.config(3, x_pix=1024, precision=abi.PREC_MIXED, schedule=abi.SCHED_WAVEFRONT)
     g = ctx.render(w.scene, w.cameras, w.integ, w.lam)
     assert len(g["cls"]) == 1024 * 1024
     assert set(np.unique(g["cls"])) <= {1, 2, 3, 4}
