@@ -16,6 +16,12 @@
 
 using namespace rtw;
 
+// Camera pass of the wavefront schedule: 1 = persistent lane groups refilled
+// from the queue head, 0 = one ray per lane group in pixel order.
+#ifndef RTW_CAMERA_PASS_PERSISTENT
+#define RTW_CAMERA_PASS_PERSISTENT 1
+#endif
+
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
@@ -479,7 +485,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   A.out_steps = d_steps;
   A.out_rays = d_rays;
   A.counters = c->counters.as<unsigned long long>();
-  A.persistent = opts->schedule != RTW_SCHED_GRID ? 1 : 0;
+  const int persistent = opts->schedule != RTW_SCHED_GRID ? 1 : 0;
   A.wavefront = opts->schedule == RTW_SCHED_WAVEFRONT ? 1 : 0;
   const bool mixed = opts->precision == RTW_PREC_MIXED;
 
@@ -514,7 +520,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   if (timed) HIPCHK(c, hipEventRecord(c->ev0, st));
   hipError_t e = hipSuccess;
   if (!A.wavefront) {
-    PassArgs Q = {nullptr, (unsigned)total, qc, nullptr, nullptr, 1};
+    PassArgs Q = {nullptr, (unsigned)total, qc, nullptr, nullptr, 1, persistent};
     e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
   } else {
     // Wavefront schedule: the lane-group kernel integrates camera rays until they
@@ -522,7 +528,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
     // the ray-per-lane vacuum kernel until they terminate or meet matter / a pole,
     // and go back; repeat until both queues are empty.  Every pass is a persistent
     // launch fed by a device queue; the host only reads the queue lengths.
-    PassArgs Q = {nullptr, (unsigned)total, qc, qbuf[0], qc + 1, 1};
+    PassArgs Q = {nullptr, (unsigned)total, qc, qbuf[0], qc + 1, 1, RTW_CAMERA_PASS_PERSISTENT};
     fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
     e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
     int which = 1;  // next pass kind
@@ -533,7 +539,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
       HIPCHK(c, hipMemcpyAsync(&nin, qc + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st));
       HIPCHK(c, hipStreamSynchronize(st));
       if (nin == 0) break;
-      PassArgs P = {qbuf[cur], nin, qc, qbuf[1 - cur], qc + 1, 0};
+      PassArgs P = {qbuf[cur], nin, qc, qbuf[1 - cur], qc + 1, 0, 1};
       fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
       LaunchInfo lp;
       e = launch_pass(which, A, dA, P, mixed, c->num_cu, st, lp);

@@ -317,8 +317,18 @@ template <int KIND> __device__ __forceinline__ void keepinbounds(double* x) {
 // log10(u) and the two columns for the scene temperature staged in LDS as
 // float2 (col floor_j, col ceil_j) per row.  Float32 as the reference, written
 // with round-to-nearest intrinsics so no FMA contraction changes the result.
+//
+// The row coordinate (log u + 16) / 0.2 is formed as q0 = a*5, r = fma(-q0, 0.2f, a),
+// q0 + r*5: for every float a with 2^-103 <= |a| < 2^125 this equals the IEEE
+// quotient a / 0.2f bit for bit (checked over all such floats; 5 = RN(1/0.2f)),
+// and outside that band both land off the table, where the lookup returns 1.2.
+__device__ __forceinline__ float gaunt_row_coord(float logu) {
+  const float a = __fadd_rn(logu, 16.0f);
+  const float q0 = __fmul_rn(a, 5.0f);
+  return __fmaf_rn(__fmaf_rn(-q0, 0.2f, a), 5.0f, q0);
+}
 __device__ __forceinline__ float gaunt_row_lookup(float logu, const float2* col, float w0, float w1, int ok_j) {
-  float frac_i = __fdiv_rn(__fadd_rn(logu, 16.0f), 0.2f);
+  float frac_i = gaunt_row_coord(logu);
   int fi = (int)floorf(frac_i);
   int ci = (int)ceilf(__fadd_rn(frac_i, 1e-3f));
   if (ok_j && fi >= 1 && ci <= 146) {
@@ -331,12 +341,29 @@ __device__ __forceinline__ float gaunt_row_lookup(float logu, const float2* col,
   return 1.2f;
 }
 
+// The same lookup for FP32 stages, on the single column gc[k] = w0 col[k].x +
+// w1 col[k].y (the column weights folded in once per render: equal to the two-
+// column form up to float rounding), with clamped indices instead of a branch.
+__device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, int ok_j) {
+  const float frac_i = gaunt_row_coord(logu);
+  const float ffl = floorf(frac_i), fce = ceilf(frac_i + 1e-3f);
+  const int fi = (int)ffl, ci = (int)fce;
+  const bool ok = ok_j && fi >= 1 && ci <= 146;
+  const int li = min(max(fi, 1), 146) - 1, hi = min(max(ci, 1), 146) - 1;
+  const float g = (fce - frac_i) * gc[li] + (frac_i - ffl) * gc[hi];
+  return ok ? g : 1.2f;
+}
+
 // Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
 struct StageSc {
   double sh;   // shift
   double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4
   double lg;   // free-free: log10(shift)
   double vac;  // 1 if no matter at this stage
+};
+// The same in FP32, for the mixed-precision stages.
+struct StageF {
+  float sh, P, lg, vac;
 };
 
 }  // namespace rtw

@@ -48,7 +48,7 @@ hipError_t launch_one(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
   const long long need = ((long long)Q.nin + gpb - 1) / gpb;
   long long grid = need;
   int bpc = 0;
-  if (A.persistent) {
+  if (Q.persistent) {
     e = persistent_grid(kern, lds, need, num_cu, grid, bpc);
     if (e != hipSuccess) return e;
   }

@@ -33,8 +33,9 @@ constexpr int BLOCK = 256;
 
 // Dormand-Prince tableau (main/64BitConstants.jl:28-37) in forward-accumulation
 // form: DPC[s][t] multiplies k_s into the sum of stage s+t (stage 7 = the b row),
-// DPE[s] is the error weight b*_s (row 9), DPB[s] the b row.
-__constant__ double DPC[8][6] = {
+// DPE[s] is the error weight b*_s (row 9).  constexpr, so that unrolled stage
+// loops get the coefficients as immediates.
+constexpr double DPC[8][6] = {
     {0, 0, 0, 0, 0, 0},
     {0, 1.0 / 5, 3.0 / 40, 44.0 / 45, 19372.0 / 6561, 9017.0 / 3168},      // k1 -> stages 2..6
     {0, 9.0 / 40, -56.0 / 15, -25360.0 / 2187, -355.0 / 33, 0},            // k2 -> 3..7 (b2 = 0)
@@ -43,7 +44,7 @@ __constant__ double DPC[8][6] = {
     {0, -5103.0 / 18656, -2187.0 / 6784, 0, 0, 0},                         // k5 -> 6..7
     {0, 11.0 / 84, 0, 0, 0, 0},                                            // k6 -> 7
     {0, 0, 0, 0, 0, 0}};
-__constant__ double DPE[8] = {0, 5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100,
+constexpr double DPE[8] = {0, 5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100,
                               1.0 / 40};
 constexpr double B1 = 35.0 / 384;
 
@@ -88,14 +89,15 @@ struct KArgs {
   int32_t* out_steps;
   double* out_rays;         // optional (total, n) column-major
   unsigned long long* counters;  // attempted, accepted, fp64, transfer steps
-  int persistent;
   int wavefront;            // rays hand frozen-vacuum stretches to vacuum_kernel
   RayPool pool;
 };
 
 // Per-launch queue arguments (by value).  A pass takes rays from qin[0..nin)
-// (or ray ids 0..nin when camera != 0, generating camera rays) through the
-// atomic head counter, and appends rays it hands over to qout / *nout.
+// (or ray ids 0..nin when camera != 0, generating camera rays) and appends rays
+// it hands over to qout / *nout.  persistent != 0: an occupancy-sized grid whose
+// lane groups pull the next ray through the atomic head counter; otherwise one
+// ray per lane group in launch order (neighbouring pixels share a wave).
 struct PassArgs {
   const unsigned* qin;
   unsigned nin;
@@ -103,6 +105,7 @@ struct PassArgs {
   unsigned* qout;
   unsigned* nout;
   int camera;
+  int persistent;
 };
 
 constexpr double FREEZE_REL = 1.0 / 72057594037927936.0;  // 2^-56
@@ -154,6 +157,27 @@ __device__ __forceinline__ StageSc make_stage(const DevScene& S, const StageOut<
   return ss;
 }
 
+template <int KIND>
+__device__ __forceinline__ StageF make_stage_f(const DevScene& S, const StageOut<float>& o) {
+  StageF ss;
+  ss.sh = o.sh;
+  ss.lg = 0.0f;
+  if (!o.matter) {
+    ss.vac = 1.0f;
+    ss.P = (-1.0f / o.sh) * 0.0f;
+  } else {
+    ss.vac = 0.0f;
+    if constexpr (KIND == KERR_FF) {
+      float q = -1.0f / o.sh;
+      ss.P = q * float(S.ff_C) * o.w / (o.sh * o.sh * o.sh);
+      ss.lg = log10_(o.sh);
+    } else {
+      ss.P = (-1.0f / o.sh) * float(S.bb_a);
+    }
+  }
+  return ss;
+}
+
 // Transfer slopes of one colour at one stage (integrators_LttM.jl:37-42 with the
 // scene's calc_radiative_coefficients).  Evaluated in T.
 template <int KIND, class T>
@@ -195,8 +219,11 @@ struct Integrator {
   int tid;
   const DevColor* scol;  // LDS
   const float2* sg;      // LDS
+  const float* sgc;      // LDS, weighted single Gaunt column (FP32 stages)
   StageSc* sst;          // LDS, 6 stage slots of this group
+  StageF* sstf;          // LDS, the same for FP32 stages
   double* cand;          // LDS candidate area (CMAX*4 per lane, stride BLOCK)
+  int ncol;
 
   // Ray state.  Only the owned geodesic components persist across a step (the full
   // vector is gathered where a step needs it), plus r and phi for the stop test.
@@ -209,10 +236,12 @@ struct Integrator {
   bool last_vac = false;              // last step accepted with every stage in vacuum
   unsigned long long n_transfer = 0;  // steps that ran the full transfer stages
 
-  __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, StageSc* st, double* cd)
-      : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sst(st), cand(cd) {}
+  __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, const float* gc,
+                        StageSc* st, StageF* stf, double* cd)
+      : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sgc(gc), sst(st), sstf(stf), cand(cd),
+        ncol(a.O.ncol) {}
 
-  __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < A.O.ncol; }
+  __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < ncol; }
   __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 4 + q) * BLOCK + tid]; }
 
   __device__ __forceinline__ void gather(const double* xo, double* X) const {
@@ -359,6 +388,57 @@ struct Integrator {
     return false;
   }
 
+  // Transfer stages 2..7 of colour slot m with FP32 slopes and FP64 stage sums,
+  // error terms and candidate (integrators_LttM.jl:109-185 restricted to the two
+  // components of the colour).  The stages are unrolled so the tableau entries
+  // are immediates, zero entries drop out and each stage sum is its own register.
+  __device__ __forceinline__ void colour_f32(int m, double hh, double atol, double rtol, double& e2) {
+    const DevColor& c = scol[lane + m * L];
+    const float uf = (float)c.uf, lc = (float)c.lc, if3 = (float)c.if3;
+    const float ES = float(EMISSION_SCALE);
+    const double T0 = sT[m], I0 = sI[m], kT1 = lT[m], kI1 = lI[m];
+    double aT[8];  // aT[u]: tau slope sum of stage u
+#pragma unroll
+    for (int u = 2; u <= 7; ++u) aT[u] = k1c(u - 2) * kT1;
+    double nI = kI1 * B1, eT = kT1 * DPE[1], eI = kI1 * DPE[1];
+    double nT = 0, yT = 0;
+#pragma unroll
+    for (int s = 2; s <= 7; ++s) {
+      const double tau = T0 + hh * aT[s];
+      if (s == 7) { nT = aT[7]; yT = tau; }
+      const StageF ss = sstf[s - 2];
+      float kTf = ss.P, kIf = ss.P;  // vacuum: the zero slopes of the reference
+      if (ss.vac == 0.0f) {
+        if constexpr (KIND == KERR_FF) {
+          const float u = uf * ss.sh;
+          const float g = gaunt_lookup_f32(lc + ss.lg, sgc, S.g_ok_j);
+          const float base = ss.P * if3 * g;
+          kTf = base * (1.0f - __expf(-u));
+          kIf = base * ES * __expf(-u - (float)tau);
+        } else {
+          kIf = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);
+        }
+      }
+      const double kT = kTf, kI = kIf;
+      if (DPE[s] != 0) {
+        eT += DPE[s] * kT;
+        eI += DPE[s] * kI;
+      }
+      if (s <= 6 && DPC[s][7 - s] != 0) nI += DPC[s][7 - s] * kI;
+#pragma unroll
+      for (int u = s + 1; u <= 7; ++u)
+        if (DPC[s][u - s] != 0) aT[u] += DPC[s][u - s] * kT;
+    }
+    const double yI = I0 + hh * nI;
+    const double dT = yT - T0 - hh * eT;
+    const double dI = yI - I0 - hh * eI;
+    // error terms: the reciprocal's last-bit error is immaterial to the controller
+    const double qT = dT * __builtin_amdgcn_rcp(atol + fmax(fabs(T0), fabs(yT)) * rtol);
+    const double qI = dI * __builtin_amdgcn_rcp(atol + fmax(fabs(I0), fabs(yI)) * rtol);
+    e2 += qI * qI + qT * qT;
+    CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+  }
+
   // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
   __device__ __forceinline__ bool step(bool& used64) {
     const double atol = A.O.atol, rtol = A.O.rtol;
@@ -400,13 +480,15 @@ struct Integrator {
       double X[8];
       gather(xo, X);
       double K[8];
-      StageSc ss;
+      // stage scalars: identical in every lane of the group
       if (!MIXED || f64) {
         StageOut<double> o;
         geo_rhs<KIND, double>(S, X, o);
 #pragma unroll
         for (int j = 0; j < 8; ++j) K[j] = o.d[j];
-        ss = make_stage<KIND, double>(S, o);
+        const StageSc ss = make_stage<KIND, double>(S, o);
+        sst[s - 2] = ss;
+        vac_all &= (ss.vac != 0 && ss.P == 0.0) ? 1 : 0;
       } else {
         float xf[8];
 #pragma unroll
@@ -415,10 +497,10 @@ struct Integrator {
         geo_rhs<KIND, float>(S, xf, o);
 #pragma unroll
         for (int j = 0; j < 8; ++j) K[j] = (double)o.d[j];
-        ss = make_stage<KIND, float>(S, o);
+        const StageF ss = make_stage_f<KIND>(S, o);
+        sstf[s - 2] = ss;
+        vac_all &= (ss.vac != 0 && ss.P == 0.0f) ? 1 : 0;
       }
-      sst[s - 2] = ss;  // identical in every lane of the group
-      vac_all &= (ss.vac != 0 && ss.P == 0.0) ? 1 : 0;
 #pragma unroll
       for (int g = 0; g < GM; ++g) {
         double k = sel8(K, lane + g * L);
@@ -457,6 +539,12 @@ struct Integrator {
         e2 += qI * qI + qT * qT;
         CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
       }
+    } else if (MIXED && !f64) {
+      n_transfer++;
+#pragma unroll
+      for (int m = 0; m < CMAX; ++m) {
+        if (col_ok(m)) colour_f32(m, hh, atol, rtol, e2);
+      }
     } else {
       n_transfer++;
 #pragma unroll
@@ -474,8 +562,7 @@ struct Integrator {
           if (s == 7) { nT = aT[0]; yT = tau; }
           const StageSc ss = sst[s - 2];
           double kT, kI;
-          if (!MIXED || f64) spec_eval<KIND, double>(S, ss, c, tau, sg, kT, kI);
-          else spec_eval<KIND, float>(S, ss, c, tau, sg, kT, kI);
+          spec_eval<KIND, double>(S, ss, c, tau, sg, kT, kI);
           eT += DPE[s] * kT;
           eI += DPE[s] * kI;
           if (s <= 6) nI += DPC[s][7 - s] * kI;
@@ -656,14 +743,21 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   float2* sg = reinterpret_cast<float2*>(scol + ncol_pad);
   StageSc* sstage = reinterpret_cast<StageSc*>(sg + 146);
   double* cand = reinterpret_cast<double*>(sstage + (BLOCK / L) * 6);
+  StageF* sstf = reinterpret_cast<StageF*>(cand + (size_t)CMAX * 4 * BLOCK);
+  float* sgc = reinterpret_cast<float*>(sstf + (BLOCK / L) * 6);
   for (int i = threadIdx.x; i < ncol; i += BLOCK) scol[i] = A.colors[i];
-  if (KIND == KERR_FF)
-    for (int i = threadIdx.x; i < 146; i += BLOCK) sg[i] = A.gcols[i];
+  if (KIND == KERR_FF) {
+    for (int i = threadIdx.x; i < 146; i += BLOCK) {
+      const float2 g = A.gcols[i];
+      sg[i] = g;
+      sgc[i] = __fadd_rn(__fmul_rn(A.S.gw0, g.x), __fmul_rn(A.S.gw1, g.y));
+    }
+  }
   __syncthreads();
 
   const int lane = threadIdx.x % L;
   const int grp = threadIdx.x / L;
-  Integrator<KIND, L, CMAX, MIXED> it(A, lane, threadIdx.x, scol, sg, sstage + grp * 6, cand);
+  Integrator<KIND, L, CMAX, MIXED> it(A, lane, threadIdx.x, scol, sg, sgc, sstage + grp * 6, sstf + grp * 6, cand);
 
   const long long groups = (long long)gridDim.x * (BLOCK / L);
   long long ray = -1;
@@ -675,7 +769,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   while (true) {
     if (need) {
       long long q;
-      if (A.persistent) {
+      if (Q.persistent) {
         unsigned v = 0;
         if (lane == 0) v = atomicAdd(Q.head, 1u);
         q = (long long)(unsigned)__shfl((int)v, 0, L);
@@ -728,7 +822,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
 
 template <int L, int CMAX> constexpr size_t rk45_lds_bytes(int ncol) {
   return sizeof(DevColor) * (size_t)((ncol + 1) & ~1) + sizeof(float2) * 146 + sizeof(StageSc) * (BLOCK / L) * 6 +
-         sizeof(double) * (size_t)CMAX * 4 * BLOCK;
+         sizeof(double) * (size_t)CMAX * 4 * BLOCK + sizeof(StageF) * (BLOCK / L) * 6 + sizeof(float) * 146;
 }
 
 }  // namespace rtw
