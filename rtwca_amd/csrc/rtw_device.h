@@ -80,6 +80,20 @@ __device__ __forceinline__ float exp_(float x) { return __expf(x); }
 __device__ __forceinline__ double log10_(double x) { return log10(x); }
 __device__ __forceinline__ float log10_(float x) { return __log10f(x); }
 
+// Reciprocal and square root: IEEE in double and in the reference's Float32 paths
+// (FAST = false); the hardware v_rcp_f32 / v_sqrt_f32 (1 ulp) for the FP32 stages
+// of the mixed-precision integrator (FAST = true).
+template <bool FAST, class T> __device__ __forceinline__ T rcp_(T x) {
+  if constexpr (FAST && sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+template <bool FAST, class T> __device__ __forceinline__ T sqrt_(T x) {
+  if constexpr (FAST && sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
+}
+// FAST arithmetic exactly for FP32 stage evaluation.
+template <class T> constexpr bool fast_math_v = sizeof(T) == 4;
+
 // Stage output of the geodesic right-hand side.
 template <class T> struct StageOut {
   T d[8];      // dx/dlambda
@@ -90,7 +104,7 @@ template <class T> struct StageOut {
 
 // Kerr Christoffel symbols contracted with k (Kerr.source/source_geometry.jl:195-243,
 // symmetric pairs summed once).  Gt is the type the symbols are evaluated in.
-template <class Gt, class T>
+template <class Gt, class T, bool FAST = false>
 __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4], T acc[4]) {
   Gt s, c;
   sc_(th, s, c);
@@ -100,7 +114,7 @@ __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4
   Gt D = r2 - r * rs + a2;
   Gt Aa = (r2 + a2) * S + rs * r * as2;
   Gt s2t = Gt(2) * s * c;
-  Gt iS = Gt(1) / S, iD = Gt(1) / D, is = Gt(1) / s;
+  Gt iS = rcp_<FAST>(S), iD = rcp_<FAST>(D), is = rcp_<FAST>(s);
   Gt iS2 = iS * iS, iS3 = iS2 * iS;
   Gt cot = c * is;
   Gt rma = r2 - ac2;
@@ -146,6 +160,7 @@ __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4
 // main/Iterators/LttM/integrators_LttM.jl:11-36).
 template <int KIND, class T>
 __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOut<T>& o) {
+  constexpr bool F = fast_math_v<T>;
   const T r = x[1], th = x[2];
   const T kt = x[4], kr = x[5], kth = x[6], kph = x[7];
   o.d[0] = kt; o.d[1] = kr; o.d[2] = kth; o.d[3] = kph;
@@ -154,8 +169,8 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     // Minkowski_Sphere_Spherical.source/source_geometry.jl:93-108
     T s, c;
     sc_(th, s, c);
-    T ir = T(1) / (r + T(NO_DIV_ZERO));
-    T cot = c / s;
+    T ir = rcp_<F>(r + T(NO_DIV_ZERO));
+    T cot = c * rcp_<F>(s);
     o.d[4] = T(0);
     o.d[5] = r * kth * kth + r * s * s * kph * kph;
     o.d[6] = T(-2) * ir * kr * kth + s * c * kph * kph;
@@ -168,12 +183,12 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     T s, c;
     sc_(th, s, c);
     const T rs = T(S.rs);
-    T ir = T(1) / r;
+    T ir = rcp_<F>(r);
     T rmrs = r - rs;
-    T Gt_tr = rs * T(0.5) * ir / rmrs;
+    T Gt_tr = rs * T(0.5) * ir * rcp_<F>(rmrs);
     T Gr_tt = T(0.5) * rs * rmrs * ir * ir * ir;
     T Gr_rr = -Gt_tr;
-    T cot = c / s;
+    T cot = c * rcp_<F>(s);
     o.d[4] = T(-2) * Gt_tr * kt * kr;
     o.d[5] = -(Gr_tt * kt * kt + Gr_rr * kr * kr - rmrs * kth * kth - rmrs * s * s * kph * kph);
     o.d[6] = -(T(2) * ir * kr * kth - s * c * kph * kph);
@@ -185,11 +200,11 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     o.matter = m;
     const T M = T(S.M);
     if (m) {
-      T ut = rsq_(T(1) - T(3) * M * ir);
-      T up = ut * sqrt(M * ir * ir * ir);
+      T ut = rcp_<F>(sqrt_<F>(T(1) - T(3) * M * ir));
+      T up = ut * sqrt_<F>(M * ir * ir * ir);
       o.sh = -(ut * (gtt * kt) + up * (gpp * kph));
     } else {
-      T ut = rsq_(T(1) - T(2) * M * ir);
+      T ut = rcp_<F>(sqrt_<F>(T(1) - T(2) * M * ir));
       o.sh = -(ut * (gtt * kt));
     }
   } else {
@@ -202,7 +217,7 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
       else
         kerr_accel<T, T>(r, th, T(S.a), T(S.rs), k4, acc);
     } else {
-      kerr_accel<T, T>(r, th, T(S.a), T(S.rs), k4, acc);
+      kerr_accel<T, T, F>(r, th, T(S.a), T(S.rs), k4, acc);
     }
     o.d[4] = acc[0]; o.d[5] = acc[1]; o.d[6] = acc[2]; o.d[7] = acc[3];
     T s, c;
@@ -212,7 +227,7 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     T Sg = r2 + a2 * c * c;
     T D = r2 - r * rs + a2;
     T A = (r2 + a2) * Sg + rs * r * a2 * s2;
-    T iSg = T(1) / Sg;
+    T iSg = rcp_<F>(Sg);
     T gtt = -(T(1) - rs * r * iSg);
     T gtp = -rs * a * r * s2 * iSg;
     T gpp = A * iSg * s2;
@@ -229,15 +244,15 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     o.matter = m;
     T ut, Om;
     if (m) {  // prograde Keplerian orbit (Kerr.source/source_model.jl:110-120)
-      T sr = sqrt(r);
+      T sr = sqrt_<F>(r);
       T r15 = r * sr;
-      T sM = sqrt(M);
-      Om = sM / (r15 + a * sM);
+      T sM = sqrt_<F>(M);
+      Om = sM * rcp_<F>(r15 + a * sM);
       T Dk = r15 - T(3) * M * sr + T(2) * a * sM;
-      ut = (r15 + a * sM) / (sqrt(r15) * sqrt(Dk));
+      ut = (r15 + a * sM) * rcp_<F>(sqrt_<F>(r15) * sqrt_<F>(Dk));
     } else {  // FIDO (:122-134)
-      ut = sqrt(A * iSg / D);
-      Om = T(2) * a * M * r / A;
+      ut = sqrt_<F>(A * iSg * rcp_<F>(D));
+      Om = T(2) * a * M * r * rcp_<F>(A);
     }
     T up = ut * Om;
     o.sh = -(ut * (gtt * kt + gtp * kph) + up * (gtp * kt + gpp * kph));

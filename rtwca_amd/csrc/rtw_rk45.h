@@ -162,17 +162,17 @@ __device__ __forceinline__ StageF make_stage_f(const DevScene& S, const StageOut
   StageF ss;
   ss.sh = o.sh;
   ss.lg = 0.0f;
+  const float q = -rcp_<true>(o.sh);
   if (!o.matter) {
     ss.vac = 1.0f;
-    ss.P = (-1.0f / o.sh) * 0.0f;
+    ss.P = q * 0.0f;
   } else {
     ss.vac = 0.0f;
     if constexpr (KIND == KERR_FF) {
-      float q = -1.0f / o.sh;
-      ss.P = q * float(S.ff_C) * o.w / (o.sh * o.sh * o.sh);
+      ss.P = q * float(S.ff_C) * o.w * rcp_<true>(o.sh * o.sh * o.sh);
       ss.lg = log10_(o.sh);
     } else {
-      ss.P = (-1.0f / o.sh) * float(S.bb_a);
+      ss.P = q * float(S.bb_a);
     }
   }
   return ss;

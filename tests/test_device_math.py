@@ -1,0 +1,56 @@
+"""Arithmetic identities that the gfx950 kernels rely on, checked on the CPU.
+
+The Gaunt-table row coordinate (log u + 16) / 0.2 (emission_functions.jl:61) is
+formed on the device as q0 = a*5, r = fma(-q0, 0.2f, a), q = fma(r, 5, q0)
+(rtw_device.h, gaunt_row_coord).  The check emulates the two FMAs exactly in
+float64 (each intermediate is exactly representable there, so rounding once to
+float32 is exactly what the fused operation does) and compares with the IEEE
+float32 quotient.
+"""
+import numpy as np
+
+F32_02 = np.float32(0.2)
+
+
+def _row_coord(a):
+    a = a.astype(np.float32)
+    q0 = (a * np.float32(5.0)).astype(np.float32)
+    r = (a.astype(np.float64) - q0.astype(np.float64) * np.float64(F32_02)).astype(np.float32)
+    return (q0.astype(np.float64) + r.astype(np.float64) * 5.0).astype(np.float32)
+
+
+def _binade(e):
+    base = np.float32(2.0) ** np.float32(e) if e > -126 else np.float32(np.ldexp(1.0, e))
+    bits = np.arange(1 << 23, dtype=np.uint32) + base.view(np.uint32)
+    return bits.view(np.float32)
+
+
+def test_gaunt_row_coord_equals_ieee_quotient_on_reachable_binades():
+    # every float in [2^-8, 2^7), both signs: log10(u) + 16 for u in 1e-24 .. 1e100
+    with np.errstate(over="ignore"):
+        for e in range(-8, 7):
+            a = _binade(e)
+            for x in (a, -a):
+                q = _row_coord(x)
+                ref = (x / F32_02).astype(np.float32)
+                assert np.array_equal(q.view(np.uint32), ref.view(np.uint32)), e
+
+
+def test_gaunt_row_coord_sampled_over_wide_band():
+    rng = np.random.default_rng(7)
+    e = rng.integers(-100, 125, size=4_000_000)
+    m = rng.random(4_000_000) + 1.0
+    a = (np.ldexp(m, e) * rng.choice([-1.0, 1.0], size=m.size)).astype(np.float32)
+    q = _row_coord(a)
+    with np.errstate(over="ignore"):
+        ref = (a / F32_02).astype(np.float32)
+    assert np.array_equal(q.view(np.uint32), ref.view(np.uint32))
+
+
+def test_emulation_detects_plain_reciprocal_multiply():
+    # the uncorrected a*5 differs from the quotient on many inputs, so the
+    # comparison above would see a dropped correction term
+    a = _binade(3)
+    ref = (a / F32_02).astype(np.float32)
+    naive = (a * np.float32(5.0)).astype(np.float32)
+    assert np.count_nonzero(naive != ref) > 1000
