@@ -11,6 +11,15 @@ renders once with the plain FP64 kernel (one lane group per ray, grid launch) an
 once with the FP64 kernels on the wavefront schedule, for the speed-ups and the
 per-pixel XYZ error of the mixed-precision image against FP64.
 
+roofline: the dominant kernel is the lane-group transfer kernel (rk45_kernel).
+Its algorithmic work per launch is (attempts with matter at some stage) x
+F_transfer + (its other attempts) x F_vacuum, with F from the oracle's counting
+scalar (tests/golden/flops_per_step.json); its time is the sum of its launches,
+each bracketed by HIP events on the launch stream inside the library
+(rtw_last_stats).  traffic: HBM bytes per launch of that kernel from the
+FETCH_SIZE / WRITE_SIZE summary in profiles/, calibrated on a plain copy
+(`--copy-probe-mib`), or null when the summary is absent.
+
 cpu_baseline: the oracle (CPU FP64 restatement of the reference integrator) on a
 fixed pixel sample, on the host's cores, threaded like LttM.jl:156-165.
 """
@@ -18,8 +27,8 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
-import time
 
 import numpy as np
 
@@ -28,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32 (AMD published)
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_1024_mixed_hbm.md")
 
 
 def parse():
@@ -40,12 +50,43 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--schedule", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this size (HBM counter calibration) and exit")
     return ap.parse_args()
 
 
-def flops_per_step():
+def flops_table():
     with open(os.path.join(ROOT, "tests", "golden", "flops_per_step.json")) as f:
-        return json.load(f)["cfg3"]["flops_per_step"]
+        return json.load(f)["cfg3"]
+
+
+def kernel_flops(st, F):
+    """Algorithmic ops of the lane-group and vacuum kernel launches of one render."""
+    g = st["steps_group"]
+    gt = min(st["steps_transfer"], g)
+    v = st["steps_attempted"] - g
+    return gt * F["flops_per_transfer_step"] + (g - gt) * F["flops_per_vacuum_step"], v * F["flops_per_vacuum_step"]
+
+
+def traffic_per_launch():
+    """HBM bytes per lane-group launch from the committed PMC summary (None if absent)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def copy_probe(mib, torch):
+    """A plain device-to-device copy of known size, for FETCH_SIZE / WRITE_SIZE calibration."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
 
 
 def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
@@ -115,14 +156,20 @@ def main():
     from rtwca_amd import abi, host, scenes
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib, torch)
+        return
     w = scenes.config(3, x_pix=a.x_pix, precision=abi.PREC_MIXED, schedule=a.schedule)
     ctx = host.Context(0)
     times, xyz, cls, stp = gpu_render(ctx, w, abi.PREC_MIXED, a.steps, a.warmup, torch)
+    st = ctx.last_stats()  # launch times and counters of the last timed render
     ms = float(np.mean(times))
     npix = xyz.shape[0]
     att = int(stp[:, 0].to(torch.int64).sum().item())
-    F = flops_per_step()
-    achieved = att * F / (ms * 1e-3) / 1e12
+    F = flops_table()
+    fg, fv = kernel_flops(st, F)
+    achieved = fg / (st["group_ms"] * 1e-3) / 1e12
+    nl = max(1, st["group_launches"])
     out = {
         "metric": "rays/s and RK45 steps/s, 1024^2 Kerr; mixed-prec vs FP64 at <=1e-3 XYZ error",
         "value": npix / (ms * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -134,8 +181,16 @@ def main():
         "rk45_steps_per_s": att / (ms * 1e-3),
         "steps_attempted": att,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_VALU, "traffic": None,
-                     "flops_per_step": F},
+                     "frac": achieved / PEAK_FP32_VALU, "traffic": traffic_per_launch(),
+                     "kernel": "rk45_kernel<KERR_FF, 8 lanes/ray, mixed> (lane-group transfer kernel)",
+                     "launches": st["group_launches"], "avg_launch_ms": st["group_ms"] / nl,
+                     "flops_per_launch": fg / nl, "steps_per_launch": st["steps_group"] / nl,
+                     "flops_per_transfer_step": F["flops_per_transfer_step"],
+                     "flops_per_vacuum_step": F["flops_per_vacuum_step"]},
+        "vacuum_kernel": {"launches": st["vacuum_launches"], "ms": st["vacuum_ms"],
+                          "steps": st["steps_attempted"] - st["steps_group"],
+                          "achieved_tflops": fv / max(1e-9, st["vacuum_ms"] * 1e-3) / 1e12},
+        "whole_render_tflops": (fg + fv) / (ms * 1e-3) / 1e12,
     }
     if not a.no_fp64:
         X = xyz.cpu().numpy()
@@ -147,8 +202,10 @@ def main():
             ref = np.abs(X64).max(axis=1)
             m = ref > 0
             rel = np.abs(X - X64).max(axis=1)[m] / ref[m]
+            st64 = ctx.last_stats()
+            f64g, f64v = kernel_flops(st64, F)
             out[tag] = {"rays_per_s": npix / (ms64 * 1e-3), "ms": ms64, "rk45_steps_per_s": att64 / (ms64 * 1e-3),
-                        "achieved_tflops": att64 * F / (ms64 * 1e-3) / 1e12,
+                        "achieved_tflops": (f64g + f64v) / (ms64 * 1e-3) / 1e12,
                         "speedup_of_mixed": ms64 / ms,
                         "mixed_xyz_rel_err_median": float(np.median(rel)) if m.any() else 0.0,
                         "mixed_xyz_rel_err_p99": float(np.percentile(rel, 99)) if m.any() else 0.0,

@@ -133,6 +133,12 @@ typedef struct rtw_stats {
   int64_t steps_accepted;
   int64_t steps_fp64;       /* MIXED: attempted steps that ran in FP64 */
   int64_t steps_transfer;   /* attempted steps with matter at some stage or nonzero transfer slopes */
+  int64_t steps_group;      /* attempted steps taken by the lane-group kernel (the rest ran one
+                               ray per lane in the vacuum kernel of the wavefront schedule) */
+  double group_ms;          /* summed device time of the lane-group kernel launches */
+  double vacuum_ms;         /* summed device time of the vacuum kernel launches */
+  int32_t group_launches;
+  int32_t vacuum_launches;
   int32_t lanes_per_ray;
   int32_t reserved;
 } rtw_stats;
@@ -185,6 +191,11 @@ int rtw_render(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cams, int
 int rtw_render_dev(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams,
                    const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
                    double* d_xyz, uint8_t* d_class, int32_t* d_steps, void* stream);
+
+/* Statistics of the last rtw_render / rtw_render_dev on ctx: step counters and
+ * per-kernel device times from HIP events recorded around every launch on the
+ * render's stream.  Waits for that stream.  No reference counterpart. */
+int rtw_last_stats(rtw_ctx* ctx, rtw_stats* stats);
 
 /* Brightness normalisation and conversion of one pose's XYZ to 16-bit sRGB,
  * returned as the (y_pix, x_pix) row-major image with the reference's

@@ -234,6 +234,7 @@ def render(scene, cam, integ, lam, nthreads=1, subset=None, init_rays=None, want
 def count_ops(scene, cam, integ, lam, subset):
     lam = np.ascontiguousarray(lam, dtype=np.float64)
     sub = np.ascontiguousarray(subset, dtype=np.int64)
-    out = np.zeros(3)
+    out = np.zeros(6)
     lib().rtwo_count_ops(_r(scene), _r(cam), _r(integ), _p(lam), len(lam), _p(sub), len(sub), _p(out))
-    return {"arith": out[0], "trans": out[1], "steps": out[2]}
+    return {"arith": out[0], "trans": out[1], "steps": out[2],
+            "arith_transfer": out[3], "trans_transfer": out[4], "steps_transfer": out[5]}

@@ -567,6 +567,7 @@ template <class R> struct RaySys {
   R g[4][4];
   R u[4];
   R fp[8];
+  bool fire_seen = false;  // some derivative of this system was evaluated in matter (op counting)
   RaySys(const Scene* s, const Opts& op, int nc, const double* f) : sc(s), o(op), ncol(nc), freq(f), n(8 + 2 * nc) {
     for (int i = 0; i < 4; ++i) {
       for (int j = 0; j < 4; ++j) {
@@ -582,6 +583,10 @@ template <class R> struct RaySys {
 
   // calc_ray_derivative! (integrators_LttM.jl:9-48)
   void deriv(const R* Ray, R* slope) {
+    {  // bookkeeping only, in plain double so it adds nothing to an op count
+      const double xd[4] = {val(Ray[0]), val(Ray[1]), val(Ray[2]), val(Ray[3])};
+      if (is_fire(*sc, xd)) fire_seen = true;
+    }
     for (int i = 0; i < 4; ++i) slope[i] = Ray[i + 4];
     christoffel(*sc, Ray, G);
     for (int i = 4; i < 8; ++i) {

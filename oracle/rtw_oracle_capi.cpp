@@ -281,7 +281,7 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
 // Arithmetic work of attempted RK45 steps, measured with the counting scalar on
 // the rays in subset: returns totals (arith ops, transcendental calls, attempted steps).
 int rtwo_count_ops(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o, const double* lam, int nlam,
-                   const int64_t* subset, int64_t nsub, double* out3) {
+                   const int64_t* subset, int64_t nsub, double* out6) {
   Scene sc = to_scene(s);
   Camera cam = to_cam(c);
   Opts op = to_opts(o);
@@ -292,7 +292,8 @@ int rtwo_count_ops(const rtw_scene* s, const rtw_camera* c, const rtw_integrator
   M4 M;
   camera_matrix(sc, cam, M);
   uint64_t ar0 = op_count().arith, tr0 = op_count().trans;
-  int64_t steps = 0;
+  int64_t steps = 0, steps_m = 0;
+  uint64_t ar_m = 0, tr_m = 0;  // attempts with matter at some stage
   std::vector<double> r8(n);
   std::vector<Cnt> ray(n);
   for (int64_t q = 0; q < nsub; ++q) {
@@ -310,16 +311,26 @@ int rtwo_count_ops(const rtw_scene* s, const rtw_camera* c, const rtw_integrator
     bool rej = false;
     double sc_ = 0;
     while (true) {
+      const uint64_t a1 = op_count().arith, t1 = op_count().trans;
+      sys.fire_seen = false;
       auto r = rkdp_step(sys, ray.data(), w, dt, rej);
       dt = r.first; rej = r.second;
+      if (sys.fire_seen) {
+        ar_m += op_count().arith - a1;
+        tr_m += op_count().trans - t1;
+        steps_m++;
+      }
       sc_ += 1;
       steps++;
       if (sys.terminate(ray.data(), op.max_steps, sc_)) break;
     }
   }
-  out3[0] = (double)(op_count().arith - ar0);
-  out3[1] = (double)(op_count().trans - tr0);
-  out3[2] = (double)steps;
+  out6[0] = (double)(op_count().arith - ar0);
+  out6[1] = (double)(op_count().trans - tr0);
+  out6[2] = (double)steps;
+  out6[3] = (double)ar_m;
+  out6[4] = (double)tr_m;
+  out6[5] = (double)steps_m;
   return 0;
 }
 

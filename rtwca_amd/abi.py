@@ -68,7 +68,9 @@ class Stats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_int64),
         ("steps_attempted", C.c_int64), ("steps_accepted", C.c_int64),
-        ("steps_fp64", C.c_int64), ("steps_transfer", C.c_int64), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
+        ("steps_fp64", C.c_int64), ("steps_transfer", C.c_int64), ("steps_group", C.c_int64),
+        ("group_ms", C.c_double), ("vacuum_ms", C.c_double), ("group_launches", C.c_int32),
+        ("vacuum_launches", C.c_int32), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -49,6 +49,14 @@ struct rtw_ctx {
   bool gaunt_loaded = false;
   std::vector<float> gaunt;  // 146 x 81 row-major
   DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs, pool;
+  // event pairs around the integrator launches of the last render (kind 0 = lane
+  // group, 1 = vacuum), read back by collect_stats
+  std::vector<hipEvent_t> pev;
+  std::vector<int> pkind;
+  int nlaunch = 0;
+  hipStream_t last_stream = nullptr;
+  long long last_rays = 0;
+  int last_lanes = 0;
 };
 
 static thread_local std::string g_create_err;
@@ -375,6 +383,7 @@ void rtw_context_destroy(rtw_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -517,11 +526,34 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   HIPCHK(c, hipMemcpyAsync(c->kargs.p, &A, sizeof(KArgs), hipMemcpyHostToDevice, st));
   const KArgs* dA = c->kargs.as<KArgs>();
   LaunchInfo li;
-  if (timed) HIPCHK(c, hipEventRecord(c->ev0, st));
+  c->nlaunch = 0;
+  c->last_stream = st;
+  c->last_rays = total;
+  (void)timed;
+  HIPCHK(c, hipEventRecord(c->ev0, st));
   hipError_t e = hipSuccess;
+  // one integrator launch between a pair of events
+  auto launch = [&](int which, const PassArgs& P, LaunchInfo& l) -> hipError_t {
+    const size_t k = (size_t)c->nlaunch;
+    if (c->pev.size() < 2 * k + 2) {
+      hipEvent_t a = nullptr, b = nullptr;
+      hipError_t ee = hipEventCreate(&a);
+      if (ee == hipSuccess) ee = hipEventCreate(&b);
+      if (ee != hipSuccess) return ee;
+      c->pev.push_back(a);
+      c->pev.push_back(b);
+      c->pkind.push_back(0);
+    }
+    c->pkind[k] = which;
+    hipError_t ee = hipEventRecord(c->pev[2 * k], st);
+    if (ee == hipSuccess) ee = launch_pass(which, A, dA, P, mixed, c->num_cu, st, l);
+    if (ee == hipSuccess) ee = hipEventRecord(c->pev[2 * k + 1], st);
+    if (ee == hipSuccess) c->nlaunch++;
+    return ee;
+  };
   if (!A.wavefront) {
     PassArgs Q = {nullptr, (unsigned)total, qc, nullptr, nullptr, 1, persistent};
-    e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
+    e = launch(0, Q, li);
   } else {
     // Wavefront schedule: the lane-group kernel integrates camera rays until they
     // terminate or their transfer state freezes in vacuum; frozen rays continue in
@@ -530,7 +562,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
     // launch fed by a device queue; the host only reads the queue lengths.
     PassArgs Q = {nullptr, (unsigned)total, qc, qbuf[0], qc + 1, 1, RTW_CAMERA_PASS_PERSISTENT};
     fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
-    e = launch_pass(0, A, dA, Q, mixed, c->num_cu, st, li);
+    e = launch(0, Q, li);
     int which = 1;  // next pass kind
     unsigned nin = 0;
     int cur = 0;    // qbuf[cur] holds the queue produced by the last pass
@@ -542,7 +574,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
       PassArgs P = {qbuf[cur], nin, qc, qbuf[1 - cur], qc + 1, 0, 1};
       fill_u32<<<1, 64, 0, st>>>(qc, 16, 0u);
       LaunchInfo lp;
-      e = launch_pass(which, A, dA, P, mixed, c->num_cu, st, lp);
+      e = launch(which, P, lp);
       cur = 1 - cur;
       which = 1 - which;
     }
@@ -552,7 +584,8 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
     }
   }
   if (e != hipSuccess) return fail(c, RTW_E_DEVICE, std::string("integrator launch: ") + hipGetErrorString(e) + " " + li.msg);
-  if (timed) HIPCHK(c, hipEventRecord(c->ev1, st));
+  HIPCHK(c, hipEventRecord(c->ev1, st));
+  c->last_lanes = li.lanes;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->rays = total;
@@ -561,18 +594,31 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   return RTW_OK;
 }
 
-static int collect_stats(rtw_ctx* c, rtw_stats* stats, bool timed) {
+// Counters and launch times of the last render; its stream must have completed.
+static int collect_stats(rtw_ctx* c, rtw_stats* stats) {
   if (!stats) return RTW_OK;
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  std::memset(stats, 0, sizeof(*stats));
+  stats->rays = c->last_rays;
+  stats->lanes_per_ray = c->last_lanes;
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
   stats->steps_attempted = (int64_t)cnt[0];
   stats->steps_accepted = (int64_t)cnt[1];
   stats->steps_fp64 = (int64_t)cnt[2];
   stats->steps_transfer = (int64_t)cnt[3];
-  if (timed) {
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    stats->kernel_ms = ms;
+  stats->steps_group = (int64_t)cnt[4];
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  stats->kernel_ms = ms;
+  for (int k = 0; k < c->nlaunch; ++k) {
+    HIPCHK(c, hipEventElapsedTime(&ms, c->pev[2 * k], c->pev[2 * k + 1]));
+    if (c->pkind[k] == 0) {
+      stats->group_ms += ms;
+      stats->group_launches++;
+    } else {
+      stats->vacuum_ms += ms;
+      stats->vacuum_launches++;
+    }
   }
   return RTW_OK;
 }
@@ -600,7 +646,7 @@ int rtw_render(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32
   if (out_class) HIPCHK(c, hipMemcpy(out_class, c->cls.p, total, hipMemcpyDeviceToHost));
   if (out_steps) HIPCHK(c, hipMemcpy(out_steps, c->steps.p, sizeof(int32_t) * 2 * total, hipMemcpyDeviceToHost));
   if (out_rays) HIPCHK(c, hipMemcpy(out_rays, c->rays.p, sizeof(double) * n * total, hipMemcpyDeviceToHost));
-  rc = collect_stats(c, stats, true);
+  rc = collect_stats(c, stats);
   if (rc) return rc;
   if (stats)
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -614,6 +660,15 @@ int rtw_render_dev(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, i
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
   return render_enqueue(c, scene, cams, n_cams, opts, lambda_nm, n_lambda, nullptr, d_xyz, d_class, d_steps, nullptr, st,
                         nullptr, false);
+}
+
+int rtw_last_stats(rtw_ctx* c, rtw_stats* stats) {
+  if (!c) return RTW_E_ARG;
+  if (!stats) return fail(c, RTW_E_ARG, "NULL stats");
+  if (!c->last_stream && c->nlaunch == 0) return fail(c, RTW_E_ARG, "no render on this context yet");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->last_stream));
+  return collect_stats(c, stats);
 }
 
 int rtw_xyz_to_srgb16(rtw_ctx* c, const double* xyz, int32_t x_pix, int32_t y_pix, uint16_t* out_rgb16) {

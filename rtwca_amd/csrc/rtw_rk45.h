@@ -88,7 +88,7 @@ struct KArgs {
   uint8_t* out_cls;
   int32_t* out_steps;
   double* out_rays;         // optional (total, n) column-major
-  unsigned long long* counters;  // attempted, accepted, fp64, transfer steps
+  unsigned long long* counters;  // attempted, accepted, fp64, transfer, lane-group steps
   int wavefront;            // rays hand frozen-vacuum stretches to vacuum_kernel
   RayPool pool;
 };
@@ -224,6 +224,7 @@ struct Integrator {
   StageF* sstf;          // LDS, the same for FP32 stages
   double* cand;          // LDS candidate area (CMAX*4 per lane, stride BLOCK)
   int ncol;
+  int gok;               // Gaunt column pair inside the table
 
   // Ray state.  Only the owned geodesic components persist across a step (the full
   // vector is gathered where a step needs it), plus r and phi for the stop test.
@@ -239,7 +240,7 @@ struct Integrator {
   __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, const float* gc,
                         StageSc* st, StageF* stf, double* cd)
       : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sgc(gc), sst(st), sstf(stf), cand(cd),
-        ncol(a.O.ncol) {}
+        ncol(a.O.ncol), gok(a.S.g_ok_j) {}
 
   __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < ncol; }
   __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 4 + q) * BLOCK + tid]; }
@@ -392,7 +393,8 @@ struct Integrator {
   // error terms and candidate (integrators_LttM.jl:109-185 restricted to the two
   // components of the colour).  The stages are unrolled so the tableau entries
   // are immediates, zero entries drop out and each stage sum is its own register.
-  __device__ __forceinline__ void colour_f32(int m, double hh, double atol, double rtol, double& e2) {
+  __device__ __forceinline__ void colour_f32(int m, const StageF* ssr, double hh, double atol, double rtol,
+                                             double& e2) {
     const DevColor& c = scol[lane + m * L];
     const float uf = (float)c.uf, lc = (float)c.lc, if3 = (float)c.if3;
     const float ES = float(EMISSION_SCALE);
@@ -406,19 +408,23 @@ struct Integrator {
     for (int s = 2; s <= 7; ++s) {
       const double tau = T0 + hh * aT[s];
       if (s == 7) { nT = aT[7]; yT = tau; }
-      const StageF ss = sstf[s - 2];
-      float kTf = ss.P, kIf = ss.P;  // vacuum: the zero slopes of the reference
-      if (ss.vac == 0.0f) {
-        if constexpr (KIND == KERR_FF) {
-          const float u = uf * ss.sh;
-          const float g = gaunt_lookup_f32(lc + ss.lg, sgc, S.g_ok_j);
-          const float base = ss.P * if3 * g;
-          kTf = base * (1.0f - __expf(-u));
-          kIf = base * ES * __expf(-u - (float)tau);
-        } else {
-          kIf = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);
-        }
+      const StageF ss = ssr[s - 2];
+      // Branch-free: matter slopes are computed for every stage and a vacuum stage
+      // selects the zero slopes of the reference, so all stages of all colour
+      // slots form one basic block the scheduler can interleave.
+      float kTm, kIm;
+      if constexpr (KIND == KERR_FF) {
+        const float u = uf * ss.sh;
+        const float g = gaunt_lookup_f32(lc + ss.lg, sgc, gok);
+        const float base = ss.P * if3 * g;
+        kTm = base * (1.0f - __expf(-u));
+        kIm = base * ES * __expf(-u - (float)tau);
+      } else {
+        kTm = ss.P;
+        kIm = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);
       }
+      const bool vac = ss.vac != 0.0f;
+      const float kTf = vac ? ss.P : kTm, kIf = vac ? ss.P : kIm;
       const double kT = kTf, kI = kIf;
       if (DPE[s] != 0) {
         eT += DPE[s] * kT;
@@ -541,9 +547,12 @@ struct Integrator {
       }
     } else if (MIXED && !f64) {
       n_transfer++;
+      StageF ssr[6];  // this step's stage scalars, read once
+#pragma unroll
+      for (int t = 0; t < 6; ++t) ssr[t] = sstf[t];
 #pragma unroll
       for (int m = 0; m < CMAX; ++m) {
-        if (col_ok(m)) colour_f32(m, hh, atol, rtol, e2);
+        if (col_ok(m)) colour_f32(m, ssr, hh, atol, rtol, e2);
       }
     } else {
       n_transfer++;
@@ -817,6 +826,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
     atomicAdd(&A.counters[1], tot_acc);
     atomicAdd(&A.counters[2], tot_64);
     atomicAdd(&A.counters[3], it.n_transfer);
+    atomicAdd(&A.counters[4], tot_att);
   }
 }
 

@@ -46,6 +46,7 @@ def _load():
     L.rtw_render.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
     L.rtw_render_dev.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P]
     L.rtw_xyz_to_srgb16.argtypes = [P, P, C.c_int32, C.c_int32, P]
+    L.rtw_last_stats.argtypes = [P, P]
     L.rtw_render_adjoint.argtypes = [P, P, P, P, P, C.c_int32, P, P, P, P]
     _lib = L
     return L
@@ -121,6 +122,12 @@ class Context:
                 "rays": None if rays is None else rays.reshape(n, npix).T.copy(),
                 "stats": {k: getattr(st, k) for k, _ in abi.Stats._fields_ if k != "reserved"},
                 "x_pix": c0.x_pix, "y_pix": ny}
+
+    def last_stats(self):
+        """rtw_last_stats: counters and per-kernel device times of the last render (waits for it)."""
+        st = abi.Stats()
+        self._check(self.L.rtw_last_stats(self.h, _ref(st)), "rtw_last_stats")
+        return {k: getattr(st, k) for k, _ in abi.Stats._fields_ if k != "reserved"}
 
     def xyz_to_srgb16(self, xyz, x_pix, y_pix):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)

@@ -52,15 +52,23 @@ def make_flops():
         npix = x_pix * x_pix
         sample = list(range(0, npix, max(1, npix // 96)))
         c = O.count_ops(w.scene, w.cameras[0], w.integ, w.lam, sample)
+        nv = c["steps"] - c["steps_transfer"]
         out["cfg%d" % idx] = {"workload": w.name, "arith_per_step": c["arith"] / c["steps"],
                               "trans_per_step": c["trans"] / c["steps"],
                               "flops_per_step": (c["arith"] + c["trans"]) / c["steps"],
-                              "sample_rays": len(sample), "sample_steps": int(c["steps"])}
+                              # attempts with matter at some stage / with every stage in vacuum
+                              "flops_per_transfer_step": (c["arith_transfer"] + c["trans_transfer"]) /
+                                                         max(1.0, c["steps_transfer"]),
+                              "flops_per_vacuum_step": (c["arith"] + c["trans"] - c["arith_transfer"] -
+                                                        c["trans_transfer"]) / max(1.0, nv),
+                              "sample_rays": len(sample), "sample_steps": int(c["steps"]),
+                              "sample_transfer_steps": int(c["steps_transfer"])}
         print(idx, out["cfg%d" % idx])
     with open(os.path.join(HERE, "flops_per_step.json"), "w") as f:
         json.dump(out, f, indent=1)
 
 
 if __name__ == "__main__":
-    make_fixtures()
+    if "--flops-only" not in sys.argv:
+        make_fixtures()
     make_flops()

@@ -177,3 +177,21 @@ def test_full_size_properties(ctx):
     assert np.all(g["steps"][:, 1] <= g["steps"][:, 0])
     assert np.all(np.isfinite(g["xyz"])) and np.all(g["xyz"] >= 0)
     assert g["stats"]["steps_attempted"] == int(g["steps"][:, 0].astype(np.int64).sum())
+
+
+def test_per_kernel_stats(ctx):
+    """rtw_last_stats: per-kernel launch counts/times and step split are consistent."""
+    w = scenes.config(3, x_pix=32, precision=abi.PREC_MIXED, schedule=abi.SCHED_WAVEFRONT)
+    g = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    s, last = g["stats"], ctx.last_stats()
+    assert last["steps_attempted"] == s["steps_attempted"] == int(g["steps"][:, 0].sum())
+    assert 0 < last["steps_group"] <= last["steps_attempted"]
+    assert 0 <= last["steps_transfer"] <= last["steps_group"]
+    assert last["group_launches"] >= 1 and last["vacuum_launches"] >= 1
+    assert last["group_ms"] > 0 and last["vacuum_ms"] > 0
+    assert last["group_ms"] + last["vacuum_ms"] <= last["kernel_ms"] * 1.001 + 0.05
+    # a single-kernel schedule runs every step in the lane-group kernel
+    w2 = scenes.config(3, x_pix=32, precision=abi.PREC_MIXED, schedule=abi.SCHED_PERSISTENT)
+    ctx.render(w2.scene, w2.cameras, w2.integ, w2.lam)
+    s2 = ctx.last_stats()
+    assert s2["steps_group"] == s2["steps_attempted"] and s2["group_launches"] == 1 and s2["vacuum_launches"] == 0
