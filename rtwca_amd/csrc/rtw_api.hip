@@ -510,8 +510,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
     double* d = reinterpret_cast<double*>(b);
     A.pool.geo = d; d += 8 * N;
     A.pool.ls = d; d += 8 * N;
-    A.pool.sI = d; d += nc * N;
-    A.pool.sT = d; d += nc * N;
+    A.pool.spec = d; d += 2 * nc * N;
     A.pool.h = d; d += N;
     int32_t* ip = reinterpret_cast<int32_t*>(d);
     A.pool.att = ip; ip += N;

@@ -94,6 +94,25 @@ template <bool FAST, class T> __device__ __forceinline__ T sqrt_(T x) {
 // FAST arithmetic exactly for FP32 stage evaluation.
 template <class T> constexpr bool fast_math_v = sizeof(T) == 4;
 
+// err^(-1/5), the factor of the step-size controller (integrators_LttM.jl:189).
+// FAST: the FP64 error is split exactly into 2^e m (frexp) and -0.2 log2(err) is
+// formed in double from e and an FP32 log2 of m, then 2^t = 2^round(t) * 2^frac
+// with an FP32 exp2 of the fraction (v_log_f32 / v_exp_f32): ~1e-7 relative.
+// 0, inf and NaN give what pow(1/err, 0.2) gives (inf, 0, NaN).  Otherwise pow.
+template <bool FAST> __device__ __forceinline__ double ctrl_factor(double err) {
+  if constexpr (FAST) {
+    if (!(err <= 1.7976931348623157e308)) return err == err ? 0.0 : err;
+    if (err == 0.0) return __builtin_inf();
+    int e;
+    const double m = frexp(err, &e);  // err = m 2^e, m in [0.5, 1)
+    const double t = -0.2 * (double)e - 0.2 * (double)__builtin_amdgcn_logf((float)m);
+    const double ti = rint(t);
+    return ldexp((double)__builtin_amdgcn_exp2f((float)(t - ti)), (int)ti);
+  } else {
+    return pow(1.0 / err, 0.2);
+  }
+}
+
 // Stage output of the geodesic right-hand side.
 template <class T> struct StageOut {
   T d[8];      // dx/dlambda
@@ -300,7 +319,9 @@ __device__ __forceinline__ bool near_singularity(const double* x, double h, doub
 }
 
 // Julia mod(x, 2pi) for x >= 0 (floored; keepinbounds! only calls it after theta >= 0).
+// x already in [0, y) is returned as is, which is what fmod gives there.
 __device__ __forceinline__ double jmod(double x, double y) {
+  if (x >= 0 && x < y) return x;
   double r = fmod(x, y);
   if (r == 0.0) return copysign(0.0, y);
   if ((r > 0) != (y > 0)) return r + y;

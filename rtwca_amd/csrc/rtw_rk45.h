@@ -60,14 +60,15 @@ __device__ __forceinline__ double k1c(int t) {
   }
 }
 
-// Ray pool of the wavefront schedule: the state of every ray between passes,
-// structure-of-arrays over the N rays of the render (component-major).
+// Ray pool of the wavefront schedule: the state of every ray between passes, one
+// record per ray (ray-major), so a lane group storing or resuming its ray touches
+// contiguous 64- and 128-byte runs and a vacuum lane reads whole 64-byte lines,
+// whatever order the queues hand the rays out in.
 enum { RAY_REJECTED = 1, RAY_DONE = 2 };
 struct RayPool {
-  double* geo;    // [8][N]  t r theta phi k^t k^r k^theta k^phi
-  double* ls;     // [8][N]  geodesic part of the last slope
-  double* sI;     // [ncol][N]  I/nu^3
-  double* sT;     // [ncol][N]  tau
+  double* geo;    // [N][8]  t r theta phi k^t k^r k^theta k^phi
+  double* ls;     // [N][8]  geodesic part of the last slope
+  double* spec;   // [N][ncol][2]  (I/nu^3, tau) per colour
   double* h;      // [N]
   int32_t* att;   // [N] attempted steps so far
   int32_t* acc;   // [N] accepted steps so far
@@ -593,7 +594,7 @@ struct Integrator {
 
     // ---- error norm and controller (integrators_LttM.jl:179-223) ----------------
     double err = sqrt(gsum<L>(e2) / (double)A.O.n);
-    double nh = 0.9 * hh * pow(1.0 / err, 0.2);
+    double nh = 0.9 * hh * ctrl_factor<MIXED>(err);
     if (fabs(nh) > max_dt) nh = -max_dt;
     bool accept;
     if (err == 0 || !isfinite(nh)) {
@@ -653,18 +654,16 @@ struct Integrator {
   }
   __device__ __forceinline__ void store(long long ray, int att, int acc, int flags) {
     const RayPool& P = A.pool;
-    const long long N = A.total;
 #pragma unroll
     for (int g = 0; g < GM; ++g) {
-      P.geo[ray + (long long)(lane + g * L) * N] = Ro[g];
-      P.ls[ray + (long long)(lane + g * L) * N] = LS[g];
+      P.geo[ray * 8 + lane + g * L] = Ro[g];
+      P.ls[ray * 8 + lane + g * L] = LS[g];
     }
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) {
       if (!col_ok(m)) continue;
       long long c = lane + m * L;
-      P.sI[ray + c * N] = sI[m];
-      P.sT[ray + c * N] = sT[m];
+      *reinterpret_cast<double2*>(P.spec + (ray * ncol + c) * 2) = make_double2(sI[m], sT[m]);
     }
     if (lane == 0) {
       P.h[ray] = h;
@@ -676,11 +675,10 @@ struct Integrator {
   // Resume a ray handed back by the vacuum kernel: its transfer slopes are zero.
   __device__ __forceinline__ void resume(long long ray, int& att, int& acc) {
     const RayPool& P = A.pool;
-    const long long N = A.total;
 #pragma unroll
     for (int g = 0; g < GM; ++g) {
-      Ro[g] = P.geo[ray + (long long)(lane + g * L) * N];
-      LS[g] = P.ls[ray + (long long)(lane + g * L) * N];
+      Ro[g] = P.geo[ray * 8 + lane + g * L];
+      LS[g] = P.ls[ray * 8 + lane + g * L];
     }
     double R[8];
     gather(Ro, R);
@@ -692,8 +690,9 @@ struct Integrator {
       lT[m] = 0;
       if (!col_ok(m)) { sI[m] = 0; sT[m] = 0; continue; }
       long long c = lane + m * L;
-      sI[m] = P.sI[ray + c * N];
-      sT[m] = P.sT[ray + c * N];
+      const double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * ncol + c) * 2);
+      sI[m] = v.x;
+      sT[m] = v.y;
     }
     h = P.h[ray];
     rejected = (P.flags[ray] & RAY_REJECTED) != 0;

@@ -30,7 +30,6 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
   const KArgs& A = *Ap;
   const DevScene& S = A.S;
   const RayPool& P = A.pool;
-  const long long N = A.total;
   const double atol = A.O.atol, rtol = A.O.rtol, max_steps = A.O.max_steps;
   const double inv_n = 1.0 / (double)A.O.n;
   unsigned long long tot_att = 0, tot_acc = 0, tot_64 = 0;
@@ -47,8 +46,8 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
       ray = Q.qin[q];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        R[j] = P.geo[ray + j * N];
-        LS[j] = P.ls[ray + j * N];
+        R[j] = P.geo[ray * 8 + j];
+        LS[j] = P.ls[ray * 8 + j];
       }
       h = P.h[ray];
       rej = (P.flags[ray] & RAY_REJECTED) != 0;
@@ -112,8 +111,8 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
       // back to the group kernel, this attempt not taken
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        P.geo[ray + j * N] = R[j];
-        P.ls[ray + j * N] = LS[j];
+        P.geo[ray * 8 + j] = R[j];
+        P.ls[ray * 8 + j] = LS[j];
       }
       P.h[ray] = h;
       P.flags[ray] = rej ? RAY_REJECTED : 0;
@@ -133,7 +132,7 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
       e2 += q * q;
     }
     double err = sqrt(e2 * inv_n);
-    double nh = 0.9 * h * pow(1.0 / err, 0.2);
+    double nh = 0.9 * h * ctrl_factor<MIXED>(err);
     if (fabs(nh) > max_dt) nh = -max_dt;
     bool accept;
     if (err == 0 || !isfinite(nh)) {
@@ -163,7 +162,7 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
     // calc_terminate without the optical-depth test (frozen tau was not thick)
     if ((double)att >= max_steps || R[1] > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) || R[3] > 8 * PI) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) P.geo[ray + j * N] = R[j];
+      for (int j = 0; j < 8; ++j) P.geo[ray * 8 + j] = R[j];
       P.h[ray] = h;
       P.att[ray] = att;
       P.acc[ray] = acc;
@@ -190,11 +189,12 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   bool thick = A.O.ncol > 0;
   for (int c = 0; c < A.O.ncol; ++c) {
     const DevColor& k = A.colors[c];
-    double I = P.sI[ray + c * N] * k.cx;
+    const double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * A.O.ncol + c) * 2);
+    double I = v.x * k.cx;
     X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
-    if (!(P.sT[ray + c * N] >= A.O.tau_thr)) thick = false;
+    if (!(v.y >= A.O.tau_thr)) thick = false;
   }
-  double r = P.geo[ray + N];
+  double r = P.geo[ray * 8 + 1];
   int cls;
   if (!isfinite(r)) cls = CLS_BOUNDARY;
   else if (A.S.kind != MINK && r <= A.S.r_horizon) cls = CLS_HORIZON;
@@ -205,10 +205,11 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   if (A.out_cls) A.out_cls[ray] = (uint8_t)cls;
   if (A.out_steps) { A.out_steps[2 * ray] = P.att[ray]; A.out_steps[2 * ray + 1] = P.acc[ray]; }
   if (A.out_rays) {
-    for (int j = 0; j < 8; ++j) A.out_rays[ray + j * N] = P.geo[ray + j * N];
+    for (int j = 0; j < 8; ++j) A.out_rays[ray + j * N] = P.geo[ray * 8 + j];
     for (int c = 0; c < A.O.ncol; ++c) {
-      A.out_rays[ray + (8 + 2 * c) * N] = P.sI[ray + c * N];
-      A.out_rays[ray + (9 + 2 * c) * N] = P.sT[ray + c * N];
+      const double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * A.O.ncol + c) * 2);
+      A.out_rays[ray + (8 + 2 * c) * N] = v.x;
+      A.out_rays[ray + (9 + 2 * c) * N] = v.y;
     }
   }
 }

@@ -54,3 +54,26 @@ def test_emulation_detects_plain_reciprocal_multiply():
     ref = (a / F32_02).astype(np.float32)
     naive = (a * np.float32(5.0)).astype(np.float32)
     assert np.count_nonzero(naive != ref) > 1000
+
+
+def _ctrl_factor(err):
+    """rtw_device.h ctrl_factor<true>: err^(-1/5) from frexp + FP32 log2 / exp2."""
+    m, e = np.frexp(err)
+    t = -0.2 * e.astype(np.float64) - 0.2 * np.log2(m.astype(np.float32)).astype(np.float64)
+    ti = np.rint(t)
+    return np.ldexp(np.exp2((t - ti).astype(np.float32)).astype(np.float64), ti.astype(np.int64))
+
+
+def test_controller_factor_matches_pow():
+    rng = np.random.default_rng(3)
+    err = np.ldexp(rng.random(2_000_000) + 0.5, rng.integers(-1070, 1020, size=2_000_000))
+    err = np.concatenate([err, [5e-324, 1e-300, 1e-10, 0.5, 1.0, 1.0 + 2e-16, 2.0, 1e10, 1e300]])
+    ref = np.power(1.0 / err, 0.2)
+    got = _ctrl_factor(err)
+    ok = np.isfinite(ref)
+    assert np.max(np.abs(got[ok] / ref[ok] - 1)) < 3e-7
+    # the exponent split is what keeps the error flat: dropping it (a float32
+    # log2 of err itself) loses the tiny and huge errors entirely
+    with np.errstate(all="ignore"):
+        naive = np.exp2(-0.2 * np.log2(err.astype(np.float32)).astype(np.float64))
+    assert not np.all(np.abs(naive[ok] / ref[ok] - 1) < 3e-7)
