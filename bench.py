@@ -52,6 +52,11 @@ def parse():
     ap.add_argument("--schedule", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (HBM counter calibration) and exit")
+    ap.add_argument("--adjoint-check", type=int, default=0,
+                    help="only run this many parameter draws of the differentiable Kerr workload: adjoint "
+                         "gradients vs central finite differences, and exit")
+    ap.add_argument("--adjoint-x-pix", type=int, default=256)
+    ap.add_argument("--adjoint-first", type=int, default=0, help="index of the first draw of --adjoint-check")
     return ap.parse_args()
 
 
@@ -87,6 +92,44 @@ def copy_probe(mib, torch):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
+
+
+def adjoint_check(ndraw, x_pix, first=0):
+    """Differentiable Kerr workload (x_pix^2, 64 wavelengths, rtol = atol = 1e-8): for each
+    draw p (uniform in the parameter boxes), L2 loss against an image at p perturbed by 5 %
+    (normal, per parameter); adjoint gradient vs central differences with h = 1e-4 in each
+    parameter's natural scale, of the recorded accepted-step map (frozen) and of fresh
+    adaptive renders (adaptive).  Prints one JSON line."""
+    from rtwca_amd import adjoint, host
+    ctx = host.Context(0)
+    rows = []
+    for i in range(first, first + ndraw):
+        rng = np.random.default_rng([20240601, i])  # draw i does not depend on how the draws are split
+        p = adjoint.draw(rng)
+        ptgt = p * (1 + 0.05 * rng.standard_normal(4))
+        tgt = adjoint.render(ctx, ptgt, x_pix, 64)["xyz"]
+        r = adjoint.gradcheck(ctx, p, tgt, x_pix, 64)
+        rows.append(r)
+        print("  draw %d p=%s frozen rel %s adaptive rel %s kept %.2f adjoint %.1fs" %
+              (i, np.array2string(p, precision=4), np.array2string(r["rel_frozen"], precision=2),
+               np.array2string(r["rel_adaptive"], precision=2), r["kept_frozen"], r["adjoint_seconds"]),
+              file=sys.stderr, flush=True)
+
+    def summary(key):
+        a = np.array([r[key] for r in rows])
+        return {name: {"median": float(np.median(a[:, k])), "p90": float(np.percentile(a[:, k], 90)),
+                       "max": float(a[:, k].max()), "within_1e-3": float((a[:, k] <= 1e-3).mean())}
+                for k, name in enumerate(adjoint.PARAMS)}
+    print(json.dumps({"adjoint_check": {
+        "workload": "kerr_thick_ff_adjoint_%d" % x_pix, "draws": ndraw, "fd_step": "1e-4 (spin, inclination "
+        "absolute; temperature, density relative)", "frozen_rel_err": summary("rel_frozen"),
+        "adaptive_rel_err": summary("rel_adaptive"),
+        "kept_pixels_median": float(np.median([r["kept_frozen"] for r in rows])),
+        "adjoint_seconds_median": float(np.median([r["adjoint_seconds"] for r in rows])),
+        "steps_per_render_median": float(np.median([r["steps"] for r in rows])),
+        "per_draw": [{"draw": first + i, "adjoint": r["adjoint_frozen"].tolist(), "fd_frozen": r["fd_frozen"].tolist(),
+                      "adjoint_adaptive_set": r["adjoint_adaptive"].tolist(), "fd_adaptive": r["fd_adaptive"].tolist(),
+                      "kept": r["kept_frozen"]} for i, r in enumerate(rows)]}}))
 
 
 def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
@@ -158,6 +201,9 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib, torch)
+        return
+    if a.adjoint_check:
+        adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first)
         return
     w = scenes.config(3, x_pix=a.x_pix, precision=abi.PREC_MIXED, schedule=a.schedule)
     ctx = host.Context(0)

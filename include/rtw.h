@@ -57,6 +57,15 @@ enum rtw_scene_kind {
  * spin/horizon/ISCO constants in Float32 (:7-37) and Float32(max_dt_scale)
  * (main/Kerr.source/source_model.jl:93-95). */
 #define RTW_SCENE_REF_F32 1
+/* RTW_SCENE_GAUNT_FP64: a smooth van Hoof Gaunt interpolation for the thick
+ * free-free disk: the same table and one-cell index offset as the reference's
+ * gaunt_ei_therm_van_Hoof (main/emission_functions.jl:54-78), in FP64 and with
+ * plain bilinear weights.  The reference rounds log u and the coordinates to
+ * Float32 (a staircase in the scene parameters) and its ceil(frac + 1e-3) gives
+ * weights summing to 2 on the last 0.001 of every cell (a jump); neither is
+ * differentiable.  Used by the differentiable workload; the FP64 kernels and the
+ * adjoint honour it, the mixed-precision FP32 stages use the reference's lookup. */
+#define RTW_SCENE_GAUNT_FP64 2
 
 typedef struct rtw_scene {
   int32_t kind;          /* rtw_scene_kind */
@@ -207,14 +216,41 @@ int rtw_xyz_to_srgb16(rtw_ctx* ctx, const double* xyz, int32_t x_pix, int32_t y_
 
 /* Gradient of L = sum over pixels and X,Y,Z of dL_dxyz * XYZ with respect to
  * (spin, camera inclination theta, disk temperature, density scale) for the
- * free-free thick-disk scene, by a hand-written discrete adjoint of the RK45
- * geodesic + transfer integration with checkpointed recomputation of the stage
- * states; the accepted step sizes are held fixed.  out_xyz (NULL allowed)
- * receives the forward image; out_grad receives 4 doubles.
+ * thick free-free disk scene, one pose, FP64: a hand-written discrete adjoint
+ * of the RK45 geodesic + transfer integration (the accepted step sizes held
+ * fixed) with checkpointed recomputation of the stage states.
+ *   dL_dxyz      3 * num_pix doubles; NULL = forward only (image and info)
+ *   out_xyz      3 * num_pix doubles, the forward image (NULL allowed)
+ *   out_grad     4 doubles: dL/d(spin, theta, T, density scale) (NULL allowed)
+ *   out_grad_pix 4 * num_pix doubles, each pixel's contribution (NULL allowed)
+ *   out_info     4 * num_pix int32: attempted and accepted steps, stage
+ *                evaluations inside matter, flags (1 = Euler pole fallback
+ *                taken: the pixel contributes no gradient) (NULL allowed)
+ * The disk's inner edge (the reference's ISCO formula, main/Kerr.source/
+ * source_geometry.jl:20-37) and the matter indicator are not differentiated.
  * No counterpart in the reference (it has no differentiation). */
 int rtw_render_adjoint(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cam,
                        const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
-                       const double* dL_dxyz, double* out_xyz, double* out_grad, rtw_stats* stats);
+                       const double* dL_dxyz, double* out_xyz, double* out_grad, double* out_grad_pix,
+                       int32_t* out_info, rtw_stats* stats);
+
+/* The recorded accepted-step map of the last rtw_render_adjoint call that
+ * computed a gradient on ctx, evaluated at new parameters: the camera ray and
+ * initial slope at the new parameters, then exactly the recorded accepted steps
+ * with their recorded sizes (no controller; the replay stops early only where a
+ * ray reaches the horizon of the new parameters or its state stops being
+ * finite).  Central
+ * differences of this map are what the adjoint differentiates, so it checks
+ * the adjoint without the step-size changes of a fresh adaptive run.  The
+ * camera grid and colours must be those of the recorded run.  out_info per
+ * pixel: theta reflections of keepinbounds, steps replayed, stage evaluations
+ * inside matter, flags -- with the step sequence fixed these change only where
+ * the map itself jumps (a stage point crossing a matter boundary, a reflection
+ * moving to another step, the horizon stop moving).  No counterpart in the
+ * reference. */
+int rtw_adjoint_replay(rtw_ctx* ctx, const rtw_scene* scene, const rtw_camera* cam,
+                       const rtw_integrator* opts, const double* lambda_nm, int32_t n_lambda,
+                       double* out_xyz, int32_t* out_info);
 
 #ifdef __cplusplus
 }

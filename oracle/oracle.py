@@ -35,6 +35,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.rtwo_gaunt.restype = C.c_float
         L.rtwo_gaunt.argtypes = [D, D]
+        L.rtwo_gaunt_f64.restype = D
+        L.rtwo_gaunt_f64.argtypes = [D, D]
         for name in ("rtwo_planck", "rtwo_circ_beta"):
             getattr(L, name).restype = D
         L.rtwo_planck.argtypes = [D, D]
@@ -70,6 +72,10 @@ def _r(x):
 
 def gaunt(gam2, u):
     return lib().rtwo_gaunt(gam2, u)
+
+
+def gaunt_f64(gam2, u):
+    return lib().rtwo_gaunt_f64(gam2, u)
 
 
 def planck(T, f):

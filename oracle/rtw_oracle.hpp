@@ -139,6 +139,7 @@ enum SceneKind {
 };
 enum SceneFlags {
   REF_F32 = 1,  // Kerr family: Float32 Christoffel, horizon/ISCO and max_dt_scale as the reference
+  GAUNT_FP64 = 2,  // thick disk: smooth Gaunt interpolation (double, plain bilinear weights)
 };
 
 struct Scene {
@@ -437,6 +438,27 @@ inline float gaunt_van_hoof(double gam2, double u) {
   return 1.2f;
 }
 
+// The smooth form for the differentiable scene: the same table and one-cell index
+// offset, in double (no Float32 rounding of log u, coordinates or weights), with
+// plain bilinear weights (ceil = floor + 1, so the weights always sum to 1: the
+// reference's ceil(frac + 1e-3) makes them sum to 2 on the last 0.001 of each
+// cell, a jump).
+inline double gaunt_van_hoof_f64(double gam2, double u) {
+  const GauntTable& t = gaunt_table();
+  const double loggam = std::log10(gam2), logu = std::log10(u);
+  const double frac_i = (logu + 16.0) / 0.2, frac_j = (loggam + 6.0) / 0.2;
+  if (!(std::fabs(frac_i) < 1e9) || !(std::fabs(frac_j) < 1e9)) return 1.2;
+  const int32_t floor_i = (int32_t)std::floor(frac_i), ceil_i = floor_i + 1;
+  const int32_t floor_j = (int32_t)std::floor(frac_j), ceil_j = floor_j + 1;
+  if ((1 <= floor_i) && (1 <= floor_j) && (ceil_i <= 146) && (ceil_j <= 81)) {
+    auto L = [&](int i, int j) { return (double)t.v[i - 1][j - 1]; };
+    const double g1 = (ceil_i - frac_i) * L(floor_i, floor_j) + (frac_i - floor_i) * L(ceil_i, floor_j);
+    const double g2 = (ceil_i - frac_i) * L(floor_i, ceil_j) + (frac_i - floor_i) * L(ceil_i, ceil_j);
+    return (ceil_j - frac_j) * g1 + (frac_j - floor_j) * g2;
+  }
+  return 1.2;
+}
+
 // Matter region test (is_fire).  Minkowski source_model.jl:253-260 (r <= 1),
 // Schwarzschild :273-280 and Kerr :73-80 (wedges); the thick disk (new) is a
 // radial band with |cos theta| <= z_cut * h_over_r.
@@ -527,7 +549,8 @@ template <class R> void radiative_coefficients(const Scene& s, const R* x, const
     if (is_fire(s, x)) {
       R T = fp[0];
       R u = R(F_FACTOR) * nu / T;
-      float gff = gaunt_van_hoof(GAM2_FACTOR / val(T), val(u));
+      const double gff = (s.flags & GAUNT_FP64) ? gaunt_van_hoof_f64(GAM2_FACTOR / val(T), val(u))
+                                                : (double)gaunt_van_hoof(GAM2_FACTOR / val(T), val(u));
       if (std::is_same<R, Cnt>::value) { op_count().arith += 14; op_count().trans += 2; }
       a_nu = R(s.absorb) * fp[1] * sqrt(R(FF_T0) / T) * cube(R(FF_NU0) / nu) * (R(1) - exp(-u)) * R((double)gff);
       j_nu = a_nu * planck(T, nu);

@@ -69,6 +69,7 @@ int rtwo_load_gaunt(const char* path) {
 }
 
 float rtwo_gaunt(double gam2, double u) { return gaunt_van_hoof(gam2, u); }
+double rtwo_gaunt_f64(double gam2, double u) { return gaunt_van_hoof_f64(gam2, u); }
 double rtwo_planck(double T, double f) { return planck<double>(T, f); }
 double rtwo_isco_ref(double a, double M, int f32) { return kerr_isco_ref(a, M, f32 != 0); }
 double rtwo_circ_beta(double r, double a, double M) { return kerr_circ_beta(r, a, M); }

@@ -21,6 +21,7 @@ SCENE_KERR_THICK_FF = 4
 SCENE_KERR_SYNCH = 5
 
 SCENE_REF_F32 = 1
+SCENE_GAUNT_FP64 = 2
 
 PREC_FP64 = 0
 PREC_MIXED = 1

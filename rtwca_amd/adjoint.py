@@ -1,0 +1,140 @@
+"""Image gradients of the thick free-free Kerr disk through the hand-written adjoint.
+
+Parameters p = (spin a, camera inclination theta [rad], disk temperature T [K],
+density scale rho_s).  `render(p)` is the FP64 forward of the adjoint (one
+ray per thread, the plain method of integrators_LttM.jl), `KerrImage` wraps
+forward and adjoint as a torch.autograd.Function, and `gradcheck` compares the
+adjoint with central finite differences of the forward.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import abi, host, scenes
+
+PARAMS = ("spin", "inclination", "temperature", "density")
+# sampling boxes of the differentiable workload (inclination in degrees)
+BOXES = {"spin": (0.5, 0.95), "inclination": (30.0, 85.0), "temperature": (5e3, 2e4), "density": (0.1, 10.0)}
+
+
+def draw(rng):
+    """One parameter draw: uniform in each box; inclination returned in radians."""
+    a = rng.uniform(*BOXES["spin"])
+    th = math.radians(rng.uniform(*BOXES["inclination"]))
+    T = rng.uniform(*BOXES["temperature"])
+    rho = rng.uniform(*BOXES["density"])
+    return np.array([a, th, T, rho])
+
+
+def workload(p, x_pix=256, n_lambda=64, rtol=1e-8, atol=1e-8, max_steps=1e4):
+    """Scene, camera and integrator of the differentiable Kerr render at parameters p:
+    the thick free-free disk (inner edge at the reference-formula ISCO of the spin,
+    Gaunt interpolation in FP64 so the image is smooth in p), camera at r = 10 and
+    theta = p[1], at rest, 60-degree field of view."""
+    a, th, T, rho = (float(v) for v in p)
+    sc = scenes.kerr_thick_ff(spin=a, temperature=T, density=rho, gaunt_fp64=True)
+    cam = abi.make_camera([0, 10, th, 0], [0, 0], 0.0, [-math.pi / 2, 0, 0], x_pix, 60, 60)
+    it = abi.make_integrator(rtol, atol, 0.1, max_steps, abi.PREC_FP64, abi.SCHED_GRID)
+    return sc, cam, it, scenes.wavelengths(n_lambda)
+
+
+def render(ctx, p, x_pix=256, n_lambda=64, dL_dxyz=None, want_grad_pix=False, **kw):
+    sc, cam, it, lam = workload(p, x_pix, n_lambda, **kw)
+    return ctx.render_adjoint(sc, cam, it, lam, dL_dxyz, want_grad_pix)
+
+
+def fd_steps(p):
+    """Central-difference steps: 1e-4 in the natural scale of each parameter
+    (absolute for spin and inclination, relative for temperature and density)."""
+    return np.array([1e-4, 1e-4, 1e-4 * p[2], 1e-4 * p[3]])
+
+
+def replay(ctx, p, x_pix, n_lambda=64, **kw):
+    sc, cam, it, lam = workload(p, x_pix, n_lambda, **kw)
+    return ctx.adjoint_replay(sc, cam, it, lam)
+
+
+def _fd(plus, minus, target, keep, h):
+    fd = np.zeros(4)
+    for k in range(4):
+        lp = ((plus[k][keep] - target[keep]) ** 2).sum()
+        lm = ((minus[k][keep] - target[keep]) ** 2).sum()
+        fd[k] = (lp - lm) / (2 * h[k])
+    return fd
+
+
+def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, **kw):
+    """Adjoint of L = sum |XYZ - target|^2 against central differences (steps
+    `fd_steps`), two ways:
+      frozen:   differences of the recorded accepted-step map (rtw_adjoint_replay),
+                which is the map the adjoint differentiates; pixels where that map
+                jumps within +-h (a stage point crosses a matter boundary or a
+                keepinbounds reflection moves to another step) are left out of L;
+      adaptive: differences of fresh adaptive renders (their step sequences adapt
+                to p, which adds noise of order tolerance / h), over the same
+                boundary-free pixels.
+    Rays that took the Euler pole step, and in the frozen map rays whose replay
+    steps inside the horizon (non-finite), are left out.  Returns dict with
+    'adjoint_frozen', 'fd_frozen', 'rel_frozen', 'adjoint', 'fd', 'rel_err' and the
+    kept pixel fractions."""
+    f0 = render(ctx, p, x_pix, n_lambda, **kw)
+    t0 = time.perf_counter()
+    g = render(ctx, p, x_pix, n_lambda, dL_dxyz=2.0 * (f0["xyz"] - target), want_grad_pix=True, **kw)
+    t_adj = time.perf_counter() - t0
+    h = fd_steps(p) if h is None else np.asarray(h, dtype=np.float64)
+    r0 = replay(ctx, p, x_pix, n_lambda, **kw)
+    keep_f = (f0["info"][:, 3] == 0) & np.isfinite(r0["xyz"]).all(axis=1)
+    keep_a = keep_f.copy()
+    pf, mf, pa, ma = [], [], [], []
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h[k]
+        rp, rm = replay(ctx, p + e, x_pix, n_lambda, **kw), replay(ctx, p - e, x_pix, n_lambda, **kw)
+        cols = [0, 2]  # reflections, matter stage evaluations (where the horizon stop falls does not matter)
+        keep_f &= (rp["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
+        keep_f &= (rm["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
+        keep_f &= np.isfinite(rp["xyz"]).all(axis=1) & np.isfinite(rm["xyz"]).all(axis=1)
+        pf.append(rp["xyz"])
+        mf.append(rm["xyz"])
+        fp, fm = render(ctx, p + e, x_pix, n_lambda, **kw), render(ctx, p - e, x_pix, n_lambda, **kw)
+        keep_a &= (fp["info"][:, 3] == 0) & (fm["info"][:, 3] == 0)
+        pa.append(fp["xyz"])
+        ma.append(fm["xyz"])
+    keep_a &= keep_f
+    out = {"replay_vs_forward": float(np.max(np.abs(r0["xyz"] - f0["xyz"]) / np.maximum(np.abs(f0["xyz"]), 1e-300))),
+           "grad_all": g["grad"], "steps": int(f0["info"][:, 0].sum()), "adjoint_seconds": t_adj}
+    for tag, keep, plus, minus in (("frozen", keep_f, pf, mf), ("adaptive", keep_a, pa, ma)):
+        adj = g["grad_pix"][keep].sum(axis=0)
+        fd = _fd(plus, minus, target, keep, h)
+        out["adjoint_" + tag] = adj
+        out["fd_" + tag] = fd
+        out["rel_" + tag] = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-300)
+        out["kept_" + tag] = float(keep.mean())
+    return out
+
+
+try:
+    import torch
+
+    class KerrImage(torch.autograd.Function):
+        """XYZ image (num_pix, 3) of the differentiable Kerr render as a function of the
+        parameter tensor (spin, inclination, temperature, density scale); backward runs
+        the adjoint.  Float64 CPU tensors."""
+
+        @staticmethod
+        def forward(fctx, params, x_pix=64, n_lambda=64):
+            p = params.detach().cpu().numpy().astype(np.float64)
+            r = render(host.context(), p, x_pix, n_lambda)
+            fctx.p = p
+            fctx.x_pix = x_pix
+            fctx.n_lambda = n_lambda
+            return torch.from_numpy(r["xyz"])
+
+        @staticmethod
+        def backward(fctx, grad_xyz):
+            g = render(host.context(), fctx.p, fctx.x_pix, fctx.n_lambda,
+                       dL_dxyz=grad_xyz.detach().cpu().numpy().astype(np.float64))
+            return torch.from_numpy(g["grad"]), None, None
+except ImportError:  # torch is optional for the ctypes path
+    KerrImage = None

@@ -1,0 +1,611 @@
+// rtw_adjoint.h -- discrete adjoint of the RK45 geodesic + transfer integration.
+//
+// L = sum over pixels and X, Y, Z of dL/dXYZ * XYZ; gradients with respect to
+// (spin, camera inclination, disk temperature, density scale) of the
+// free-free thick disk, in FP64, one ray per thread.
+//
+// Forward: the plain method of integrators_LttM.jl:54-224 (FP64 stages, the
+// reference's forward stage-sum order, averaged-slope FSAL, pow controller),
+// run twice: once to count each ray's accepted steps, once to record the
+// accepted step sizes and a checkpoint (state, last slope) every K accepted
+// steps.  The accepted step map with its step size held fixed is
+//     y' = keepinbounds(y + h sum_{i<=6} b_i k_i),   k1' = sum_{i<=6} b_i k_i,
+// k_1 = k1, k_i = f(y + h sum_{j<i} a_ij k_j); k_7 and the controller only
+// choose h and are not differentiated.  Rejected attempts leave (y, k1)
+// unchanged.
+//
+// Reverse: per segment of K steps, replay from its checkpoint into a per-ray
+// buffer, then walk the steps backwards.  For each step, with
+// Lambda_s = h lambda_y' + lambda_k' and
+//     lambda_{k_i} = b_i Lambda_s + sum_{i<j<=6} h a_ji lambda_{X_j},
+//     lambda_{X_i} = J_f(X_i)^T lambda_{k_i},
+//     lambda_y = lambda_y' + sum_j lambda_{X_j},
+//     lambda_k1 = b_1 Lambda_s + sum_j h a_j1 lambda_{X_j}.
+// The transfer slopes of a colour depend on its own tau only through
+// exp(-tau) and not on I, so lambda_{X_j}[I] = 0, lambda_{k_i}[I] = b_i
+// Lambda_s[I] and lambda_{X_j}[tau] = -kI_j lambda_{k_j}[I]; their derivatives
+// with respect to the stage's frequency shift, rho^2 and the temperature are
+// written out below.  The geodesic Jacobian rows (and d/dspin) come from the
+// geodesic right-hand side evaluated on 7-tangent dual numbers.  keepinbounds
+// negates theta and k^theta on a reflection, so the adjoint flips the same two
+// components.  At the start, k1 = f(y0) and y0 is the camera ray, whose
+// derivatives with respect to spin and inclination enter as d(camera matrix)/dp
+// applied to the pixel's local direction.
+//
+// Held fixed: the accepted step sizes and the matter indicator (the disk's inner
+// edge follows the spin through the reference's ISCO formula, a boundary the
+// adjoint does not move).  Rays that took the Euler pole fallback are flagged and
+// contribute no gradient.
+#pragma once
+#include "rtw_device.h"
+#include "rtw_rk45.h"
+#include "rtw_adjoint_api.h"
+#include "rtw_dual.h"
+
+namespace rtw {
+
+// Tableau entry a_ji: coefficient of k_i in stage j (j = 7 gives b_i).
+__device__ __forceinline__ double dp_a(int j, int i) { return i == 1 ? k1c(j - 2) : DPC[i][j - i]; }
+
+// ---------------------------------------------------------------------------
+// one accepted-or-rejected attempt, shared by the forward and the replay
+// ---------------------------------------------------------------------------
+
+// Geodesic stages 2..7 from (R, LS): stage points X (2..6 kept when Xs != null),
+// slopes, stage scalars, stage-7 point yo and averaged slope ns.
+template <int KIND, bool ERR>
+__device__ __forceinline__ void geo_stages(const DevScene& S, const double* R, const double* LS, double hh,
+                                           StageSc* ss, double* wv, double (*Xs)[8], double* yo, double* ns,
+                                           double* eg, int& matter) {
+  double acc[8][6];
+#pragma unroll
+  for (int g = 0; g < 8; ++g) {
+#pragma unroll
+    for (int t = 0; t < 6; ++t) acc[g][t] = k1c(t) * LS[g];
+    if (ERR) eg[g] = DPE[1] * LS[g];
+  }
+#pragma unroll
+  for (int s = 2; s <= 7; ++s) {
+    double X[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      X[g] = R[g] + hh * acc[g][0];
+      if (s == 7) { ns[g] = acc[g][0]; yo[g] = X[g]; }
+    }
+    if (Xs && s <= 6) {
+#pragma unroll
+      for (int g = 0; g < 8; ++g) Xs[s - 2][g] = X[g];
+    }
+    if (s == 7 && !ERR) break;  // k_7 only enters the error estimate
+    StageOut<double> o;
+    geo_rhs<KIND, double>(S, X, o);
+    matter += o.matter ? 1 : 0;
+    ss[s - 2] = make_stage<KIND, double>(S, o);
+    if (wv) wv[s - 2] = o.w;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const double k = o.d[g];
+      if (ERR) eg[g] += DPE[s] * k;
+#pragma unroll
+      for (int t = 1; t < 6; ++t) acc[g][t] += DPC[s][t] * k;
+#pragma unroll
+      for (int t = 0; t < 5; ++t) acc[g][t] = acc[g][t + 1];
+      acc[g][5] = 0;
+    }
+  }
+}
+
+// Transfer stages of one colour (the FP64 path of the lane-group kernel).
+template <int KIND, bool ERR>
+__device__ __forceinline__ void colour_stages(const DevScene& S, const StageSc* ss, const DevColor& c,
+                                              const float2* gcol, double hh, double I0, double T0, double kI1,
+                                              double kT1, double atol, double rtol, double& yI, double& yT,
+                                              double& nI, double& nT, double& e2) {
+  double aT[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) aT[t] = k1c(t) * kT1;
+  nI = kI1 * B1;
+  double eT = kT1 * DPE[1], eI = kI1 * DPE[1];
+  nT = 0;
+  yT = 0;
+#pragma unroll
+  for (int s = 2; s <= 7; ++s) {
+    const double tau = T0 + hh * aT[0];
+    if (s == 7) {
+      nT = aT[0];
+      yT = tau;
+      if (!ERR) break;
+    }
+    double kT, kI;
+    spec_eval<KIND, double>(S, ss[s - 2], c, tau, gcol, kT, kI);
+    if (ERR) {
+      eT += DPE[s] * kT;
+      eI += DPE[s] * kI;
+    }
+    if (s <= 6) nI += DPC[s][7 - s] * kI;
+#pragma unroll
+    for (int t = 1; t < 6; ++t) aT[t] += DPC[s][t] * kT;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) aT[t] = aT[t + 1];
+    aT[5] = 0;
+  }
+  yI = I0 + hh * nI;
+  if (ERR) {
+    const double dT = yT - T0 - hh * eT;
+    const double dI = yI - I0 - hh * eI;
+    const double qT = dT / (atol + fmax(fabs(T0), fabs(yT)) * rtol);
+    const double qI = dI / (atol + fmax(fabs(I0), fabs(yI)) * rtol);
+    e2 += qI * qI + qT * qT;
+  }
+}
+
+// Full derivative at R (geodesic slope into LS, colour slopes into W).
+template <int KIND>
+__device__ __forceinline__ void full_deriv_thread(const AdjArgs& A, const double* R, double* LS, double* W, int nc,
+                                                  int& matter) {
+  StageOut<double> o;
+  geo_rhs<KIND, double>(A.S, R, o);
+  matter += o.matter ? 1 : 0;
+#pragma unroll
+  for (int g = 0; g < 8; ++g) LS[g] = o.d[g];
+  const StageSc ss = make_stage<KIND, double>(A.S, o);
+  for (int c = 0; c < nc; ++c) {
+    double kT, kI;
+    spec_eval<KIND, double>(A.S, ss, A.colors[c], W[4 * c + 1], A.gcols, kT, kI);
+    W[4 * c + 2] = kI;
+    W[4 * c + 3] = kT;
+  }
+}
+
+template <int KIND> __device__ __forceinline__ bool stop_thread(const AdjArgs& A, const double* R, const double* W,
+                                                                int nc, double max_steps, double step_count) {
+  const DevScene& S = A.S;
+  if (step_count >= max_steps || R[1] > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) || R[3] > 8 * PI)
+    return true;
+  bool thick = nc > 0;
+  for (int c = 0; c < nc; ++c)
+    if (!(W[4 * c + 1] >= A.O.tau_thr)) thick = false;
+  return thick;
+}
+
+// ---------------------------------------------------------------------------
+// forward: count (TAPE = false) or record (TAPE = true)
+// ---------------------------------------------------------------------------
+template <int KIND, bool TAPE>
+__global__ void __launch_bounds__(64) adj_forward(const AdjArgs* __restrict__ Ap) {
+  const AdjArgs& A = *Ap;
+  const long long ray = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (ray >= A.npix) return;
+  const DevScene& S = A.S;
+  const int nc = A.O.ncol, n = A.O.n, K = A.K;
+  const double atol = A.O.atol, rtol = A.O.rtol, max_steps = A.O.max_steps;
+  double* W = A.work + ray * 8 * nc;  // (I, tau, kI, kT) per colour
+  double* Cd = W + 4 * nc;            // candidates (yI, yT, nI, nT)
+  double R[8], LS[8];
+  camera_ray(A.cam, ray, A.x_pix, R);
+  for (int c = 0; c < 4 * nc; ++c) W[c] = 0;
+  int matter = 0, flags = 0, att = 0, acc = 0;
+  full_deriv_thread<KIND>(A, R, LS, W, nc, matter);
+  double h = -pad_max_dt<KIND>(S, R);
+  bool rej = false;
+  auto checkpoint = [&](long long slot) {
+    double* q = A.ckpt + slot * 2 * n;
+    for (int g = 0; g < 8; ++g) { q[g] = R[g]; q[n + g] = LS[g]; }
+    for (int c = 0; c < nc; ++c) {
+      q[8 + 2 * c] = W[4 * c];
+      q[9 + 2 * c] = W[4 * c + 1];
+      q[n + 8 + 2 * c] = W[4 * c + 2];
+      q[n + 9 + 2 * c] = W[4 * c + 3];
+    }
+  };
+  if (TAPE) checkpoint(A.off_c[ray]);
+  while (true) {
+    bool accept;
+    const double max_dt = pad_max_dt<KIND>(S, R);
+    double ts;
+    bool euler_done = false;
+    if (near_singularity<KIND>(R, h, atol, ts)) {
+      // Euler fallback (integrators_LttM.jl:62-105); the ray leaves the gradient
+      flags |= ADJ_EULER;
+      for (int g = 0; g < 8; ++g) R[g] -= ts * LS[g];
+      for (int c = 0; c < nc; ++c) { W[4 * c] -= ts * W[4 * c + 2]; W[4 * c + 1] -= ts * W[4 * c + 3]; }
+      keepinbounds<KIND>(R);
+      for (int it = 0; it <= 36; ++it) {
+        if (stop_thread<KIND>(A, R, W, nc, 2, 1)) {
+          if (it == 36) { euler_done = true; break; }
+          continue;
+        }
+        double d[8];
+        full_deriv_thread<KIND>(A, R, d, W, nc, matter);
+        for (int g = 0; g < 8; ++g) LS[g] = d[g];
+        if (it == 36) break;
+        for (int g = 0; g < 8; ++g) R[g] -= ts * d[g];
+        for (int c = 0; c < nc; ++c) { W[4 * c] -= ts * W[4 * c + 2]; W[4 * c + 1] -= ts * W[4 * c + 3]; }
+        keepinbounds<KIND>(R);
+      }
+      if (!euler_done) h = -pad_max_dt<KIND>(S, R);
+    }
+    double hh = h;
+    if (euler_done) {
+      accept = true;
+    } else {
+      StageSc ss[6];
+      double yo[8], ns[8], eg[8];
+      geo_stages<KIND, true>(S, R, LS, hh, ss, nullptr, nullptr, yo, ns, eg, matter);
+      double e2 = 0;
+      for (int g = 0; g < 8; ++g) {
+        const double dl = yo[g] - R[g] - hh * eg[g];
+        const double q = dl / (atol + fmax(fabs(R[g]), fabs(yo[g])) * rtol);
+        e2 += q * q;
+      }
+      for (int c = 0; c < nc; ++c) {
+        double yI, yT, nI, nT;
+        colour_stages<KIND, true>(S, ss, A.colors[c], A.gcols, hh, W[4 * c], W[4 * c + 1], W[4 * c + 2],
+                                  W[4 * c + 3], atol, rtol, yI, yT, nI, nT, e2);
+        Cd[4 * c] = yI; Cd[4 * c + 1] = yT; Cd[4 * c + 2] = nI; Cd[4 * c + 3] = nT;
+      }
+      const double err = sqrt(e2 / (double)n);
+      double nh = 0.9 * hh * ctrl_factor<false>(err);
+      if (fabs(nh) > max_dt) nh = -max_dt;
+      if (err == 0 || !isfinite(nh)) {
+        accept = true;
+        nh = hh;
+      } else if (err > 1) {
+        accept = false;
+        if (rej && fabs(nh) >= fabs(hh)) nh = hh / 2;
+      } else {
+        accept = true;
+        if (fabs(nh) > 5 * fabs(hh)) nh = 5 * hh;
+      }
+      if (accept) {
+        keepinbounds<KIND>(yo);
+        for (int g = 0; g < 8; ++g) { R[g] = yo[g]; LS[g] = ns[g]; }
+        for (int c = 0; c < 4 * nc; ++c) W[c] = Cd[c];
+      } else {
+        keepinbounds<KIND>(R);
+      }
+      h = nh;
+      rej = !accept;
+    }
+    att++;
+    if (accept) {
+      if (TAPE && acc < A.tape_info[4 * ray + 1]) {
+        A.htape[A.off_h[ray] + acc] = hh;
+        if ((acc + 1) % K == 0) checkpoint(A.off_c[ray] + (acc + 1) / K);
+      }
+      acc++;
+    }
+    if (stop_thread<KIND>(A, R, W, nc, max_steps, (double)att)) break;
+  }
+  double X = 0, Y = 0, Z = 0;
+  for (int c = 0; c < nc; ++c) {
+    const DevColor& k = A.colors[c];
+    const double I = W[4 * c] * k.cx;
+    X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
+  }
+  if (A.xyz) { A.xyz[3 * ray] = X; A.xyz[3 * ray + 1] = Y; A.xyz[3 * ray + 2] = Z; }
+  if (!TAPE) {
+    A.info[4 * ray] = att;
+    A.info[4 * ray + 1] = acc;
+    A.info[4 * ray + 2] = matter;
+    A.info[4 * ray + 3] = flags;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reverse
+// ---------------------------------------------------------------------------
+
+// Replay one accepted step (fixed h) from slot q into out; returns the number of
+// stage evaluations inside matter, and adds keepinbounds' theta reflections to *refl.
+template <int KIND> __device__ __forceinline__ int replay_step(const AdjArgs& A, const double* q, double* out,
+                                                               double hh, int* refl = nullptr) {
+  const int nc = A.O.ncol, n = A.O.n;
+  StageSc ss[6];
+  double yo[8], ns[8];
+  int matter = 0;
+  geo_stages<KIND, false>(A.S, q, q + n, hh, ss, nullptr, nullptr, yo, ns, nullptr, matter);
+  const int par = keepinbounds_parity<KIND>(yo);
+  if (refl) *refl += par;
+  for (int g = 0; g < 8; ++g) { out[g] = yo[g]; out[n + g] = ns[g]; }
+  for (int c = 0; c < nc; ++c) {
+    double yI, yT, nI, nT, e2 = 0;
+    colour_stages<KIND, false>(A.S, ss, A.colors[c], A.gcols, hh, q[8 + 2 * c], q[9 + 2 * c], q[n + 8 + 2 * c],
+                               q[n + 9 + 2 * c], 0, 0, yI, yT, nI, nT, e2);
+    out[8 + 2 * c] = yI;
+    out[9 + 2 * c] = yT;
+    out[n + 8 + 2 * c] = nI;
+    out[n + 9 + 2 * c] = nT;
+  }
+  return matter;
+}
+
+// The recorded accepted-step map at the current parameters: camera ray, k1 = f(y0),
+// then every recorded accepted step with its recorded size.  Central differences of
+// this map are what the reverse sweep differentiates.
+template <int KIND>
+__global__ void __launch_bounds__(64) adj_replay(const AdjArgs* __restrict__ Ap, long long ray0, long long nrays) {
+  const AdjArgs& A = *Ap;
+  const long long local = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (local >= nrays) return;
+  const long long ray = ray0 + local;
+  const int nc = A.O.ncol, n = A.O.n;
+  const int32_t* ti = A.tape_info + 4 * ray;
+  int32_t* out = A.info + 4 * ray;
+  if (ti[3] & ADJ_EULER) {
+    A.xyz[3 * ray] = A.xyz[3 * ray + 1] = A.xyz[3 * ray + 2] = 0;
+    out[0] = out[1] = out[2] = 0;
+    out[3] = ADJ_EULER;
+    return;
+  }
+  const int nacc = ti[1];
+  double* cur = A.seg + local * (long long)A.K * 2 * n;
+  double* nxt = cur + 2 * n;
+  camera_ray(A.cam, ray, A.x_pix, cur);
+  int matter = 0;
+  {
+    StageOut<double> o;
+    geo_rhs<KIND, double>(A.S, cur, o);
+    matter += o.matter ? 1 : 0;
+    for (int g = 0; g < 8; ++g) cur[n + g] = o.d[g];
+    const StageSc ss = make_stage<KIND, double>(A.S, o);
+    for (int c = 0; c < nc; ++c) {
+      double kT, kI;
+      spec_eval<KIND, double>(A.S, ss, A.colors[c], 0.0, A.gcols, kT, kI);
+      cur[8 + 2 * c] = 0;
+      cur[9 + 2 * c] = 0;
+      cur[n + 8 + 2 * c] = kI;
+      cur[n + 9 + 2 * c] = kT;
+    }
+  }
+  const long long oh = A.off_h[ray];
+  int done = 0, refl = 0;
+  for (int j = 0; j < nacc; ++j) {
+    int rj = 0;
+    const int mj = replay_step<KIND>(A, cur, nxt, A.htape[oh + j], &rj);
+    // At other parameters a horizon-bound ray's last recorded steps can reach past
+    // the (moved) horizon: stop there as the forward's horizon test does, keeping
+    // the last finite state (these steps are in vacuum).
+    bool bad = !isfinite(nxt[1]);
+    for (int c = 0; c < nc && !bad; ++c) bad = !isfinite(nxt[8 + 2 * c]) || !isfinite(nxt[9 + 2 * c]);
+    if (bad) break;
+    matter += mj;
+    refl += rj;
+    done++;
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+    if (KIND != MINK && cur[1] <= A.S.r_horizon) break;
+  }
+  double X = 0, Y = 0, Z = 0;
+  for (int c = 0; c < nc; ++c) {
+    const DevColor& k = A.colors[c];
+    const double I = cur[8 + 2 * c] * k.cx;
+    X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
+  }
+  A.xyz[3 * ray] = X; A.xyz[3 * ray + 1] = Y; A.xyz[3 * ray + 2] = Z;
+  out[0] = refl;
+  out[1] = done;
+  out[2] = matter;
+  out[3] = 0;
+}
+
+// Adjoint of the transfer slopes of one colour at one stage with respect to the
+// stage's shift and rho^2 and to the temperature (thick free-free disk):
+//   kT = P if3 g (1 - e^-u),  kI = P if3 g ES e^(-u - tau),
+//   P = -C rho^2 / sh^4,  C = kappa sqrt(T0 / T),  u = uf sh,  uf = F f / T,
+//   g = g_ff(log10 u, log10 gamma^2), gamma^2 = Ry / (k T).
+template <int KIND>
+__device__ __forceinline__ void colour_stage_adjoint(const DevScene& S, const StageSc& ss, const DevColor& c,
+                                                     const float2* gcol, double tau, double lkT, double lkI,
+                                                     double& lsh, double& lw, double& gT) {
+  if (KIND != KERR_FF || ss.vac != 0) return;
+  constexpr double LN10 = 2.302585092994046;
+  const double sh = ss.sh, P = ss.P, T = S.T;
+  const double u = c.uf * sh, em = exp(-u), E = exp(-u - tau);
+  double g, dgu, dgj;
+  if (S.flags & SCENE_GAUNT_FP64) {
+    g = gaunt_row_lookup_f64(c.lc + ss.lg, gcol, S.gw0d, S.gw1d, S.g_ok_j);
+    gaunt_slopes_f64(c.lc + ss.lg, gcol, S.gw0d, S.gw1d, S.g_ok_j, dgu, dgj);
+  } else {
+    const float logu = (float)(c.lc + ss.lg);
+    g = (double)gaunt_row_lookup(logu, gcol, S.gw0, S.gw1, S.g_ok_j);
+    gaunt_slopes(logu, gcol, S.gw0, S.gw1, S.g_ok_j, dgu, dgj);
+  }
+  const double ES = EMISSION_SCALE, if3 = c.if3;
+  const double base = P * if3 * g;
+  const double kI = base * ES * E;
+  const double dbase_dsh = if3 * (-4.0 * P / sh * g + P * dgu / (sh * LN10));
+  const double dkT_dsh = dbase_dsh * (1 - em) + base * em * c.uf;
+  const double dkI_dsh = dbase_dsh * ES * E - kI * c.uf;
+  const double Pw = -S.ff_C / (sh * sh * sh * sh);
+  const double dkT_dw = Pw * if3 * g * (1 - em), dkI_dw = Pw * if3 * g * ES * E;
+  const double dbase_dT = if3 * (-P / (2 * T) * g - P * (dgu + dgj) / (T * LN10));
+  const double dkT_dT = dbase_dT * (1 - em) - base * em * u / T;
+  const double dkI_dT = dbase_dT * ES * E + kI * u / T;
+  lsh += dkT_dsh * lkT + dkI_dsh * lkI;
+  lw += dkT_dw * lkT + dkI_dw * lkI;
+  gT += dkT_dT * lkT + dkI_dT * lkI;
+}
+
+// lambda_X (geodesic) += J_f(X)^T lk + dsh/dX lsh + drho^2/dX lw; returns the
+// spin derivative sum_comp lk dk/da + lsh dsh/da.
+template <int KIND>
+__device__ __forceinline__ double geo_vjp(const DevScene& S, const double* X, const double* lk, double lsh, double lw,
+                                          double* lX) {
+  using D = Dual<7>;
+  D x[8];
+  x[0] = D(X[0]);
+  x[1] = D::seed(X[1], 0);
+  x[2] = D::seed(X[2], 1);
+  x[3] = D(X[3]);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) x[4 + m] = D::seed(X[4 + m], 2 + m);
+  const D a = D::seed(S.a, 6);
+  StageOut<D> o;
+  kerr_rhs<KIND, D>(S, a, x, o);
+  double t[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    double v = lsh * o.sh.d[k] + lw * o.w.d[k];
+#pragma unroll
+    for (int comp = 0; comp < 8; ++comp) v += lk[comp] * o.d[comp].d[k];
+    t[k] = v;
+  }
+  lX[1] += t[0];
+  lX[2] += t[1];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) lX[4 + m] += t[2 + m];
+  return t[6];
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(64) adj_reverse(const AdjArgs* __restrict__ Ap, long long ray0, long long nrays) {
+  const AdjArgs& A = *Ap;
+  const long long local = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (local >= nrays) return;
+  const long long ray = ray0 + local;
+  const DevScene& S = A.S;
+  const int nc = A.O.ncol, n = A.O.n, K = A.K;
+  double* G = A.grad + 4 * ray;
+  const int nacc = A.tape_info[4 * ray + 1];
+  if (A.tape_info[4 * ray + 3] & ADJ_EULER) {
+    G[0] = G[1] = G[2] = G[3] = 0;
+    return;
+  }
+  double* lam = A.lam + local * 4 * nc;
+  const double g0 = A.dL[3 * ray], g1 = A.dL[3 * ray + 1], g2 = A.dL[3 * ray + 2];
+  for (int c = 0; c < nc; ++c) {
+    const DevColor& k = A.colors[c];
+    lam[4 * c] = k.cx * (g0 * k.wx + g1 * k.wy + g2 * k.wz);  // dL/dI_c
+    lam[4 * c + 1] = 0;
+    lam[4 * c + 2] = 0;
+    lam[4 * c + 3] = 0;
+  }
+  double lyg[8], lkg[8];
+  for (int g = 0; g < 8; ++g) { lyg[g] = 0; lkg[g] = 0; }
+  double ga = 0, gT = 0, grho = 0;
+  const double rho_s = S.density;
+  double* seg = A.seg + local * (long long)K * 2 * n;  // per-chunk segment buffer
+  const long long oh = A.off_h[ray], oc = A.off_c[ray];
+  const int nseg = (nacc + K - 1) / K;
+  for (int m = nseg - 1; m >= 0; --m) {
+    const int n0 = m * K, len = min(K, nacc - n0);
+    const double* cp = A.ckpt + (oc + m) * 2 * n;
+    for (int i = 0; i < 2 * n; ++i) seg[i] = cp[i];
+    for (int j = 0; j + 1 < len; ++j) replay_step<KIND>(A, seg + j * 2 * n, seg + (j + 1) * 2 * n, A.htape[oh + n0 + j]);
+    for (int j = len - 1; j >= 0; --j) {
+      const double hh = A.htape[oh + n0 + j];
+      const double* y = seg + j * 2 * n;
+      const double* k1 = y + n;
+      // recompute the geodesic stages
+      StageSc ss[6];
+      double wv[6], Xs[5][8], yo[8], ns[8];
+      int matter = 0;
+      geo_stages<KIND, false>(S, y, k1, hh, ss, wv, Xs, yo, ns, nullptr, matter);
+      double lX7[8];
+      const int par = keepinbounds_parity<KIND>(yo);
+      for (int g = 0; g < 8; ++g) lX7[g] = lyg[g];
+      if (par) { lX7[2] = -lX7[2]; lX7[6] = -lX7[6]; }
+      // colours
+      double lsh[5] = {0, 0, 0, 0, 0}, lw[5] = {0, 0, 0, 0, 0};
+      for (int c = 0; c < nc; ++c) {
+        const DevColor col = A.colors[c];
+        const double T0 = y[9 + 2 * c], kT1 = k1[9 + 2 * c];
+        // forward: tau and slopes at stages 2..6
+        double taus[5], kIs[5], aT[6];
+        for (int t = 0; t < 6; ++t) aT[t] = k1c(t) * kT1;
+#pragma unroll
+        for (int s = 2; s <= 6; ++s) {
+          const double tau = T0 + hh * aT[0];
+          double kT, kI;
+          spec_eval<KIND, double>(S, ss[s - 2], col, tau, A.gcols, kT, kI);
+          taus[s - 2] = tau;
+          kIs[s - 2] = kI;
+#pragma unroll
+          for (int t = 1; t < 6; ++t) aT[t] += DPC[s][t] * kT;
+#pragma unroll
+          for (int t = 0; t < 5; ++t) aT[t] = aT[t + 1];
+          aT[5] = 0;
+        }
+        const double lyI = lam[4 * c], lyT = lam[4 * c + 1], lkI = lam[4 * c + 2], lkT = lam[4 * c + 3];
+        const double SsI = hh * lyI + lkI, SsT = hh * lyT + lkT;
+        double lXT[5];
+#pragma unroll
+        for (int i = 6; i >= 2; --i) {
+          double lkTi = dp_a(7, i) * SsT;
+#pragma unroll
+          for (int jj = i + 1; jj <= 6; ++jj) lkTi += hh * dp_a(jj, i) * lXT[jj - 2];
+          const double lkIi = dp_a(7, i) * SsI;
+          lXT[i - 2] = -kIs[i - 2] * lkIi;
+          colour_stage_adjoint<KIND>(S, ss[i - 2], col, A.gcols, taus[i - 2], lkTi, lkIi, lsh[i - 2], lw[i - 2], gT);
+        }
+        double lkT1 = B1 * SsT, sumX = 0;
+#pragma unroll
+        for (int jj = 2; jj <= 6; ++jj) {
+          lkT1 += hh * dp_a(jj, 1) * lXT[jj - 2];
+          sumX += lXT[jj - 2];
+        }
+        lam[4 * c] = lyI;
+        lam[4 * c + 1] = lyT + sumX;
+        lam[4 * c + 2] = B1 * SsI;
+        lam[4 * c + 3] = lkT1;
+      }
+      // geodesic
+      double Ss[8], lXg[5][8];
+      for (int g = 0; g < 8; ++g) Ss[g] = hh * lX7[g] + lkg[g];
+      for (int i = 6; i >= 2; --i) {
+        double lk[8];
+        for (int g = 0; g < 8; ++g) {
+          double v = dp_a(7, i) * Ss[g];
+          for (int jj = i + 1; jj <= 6; ++jj) v += hh * dp_a(jj, i) * lXg[jj - 2][g];
+          lk[g] = v;
+          lXg[i - 2][g] = 0;
+        }
+        ga += geo_vjp<KIND>(S, Xs[i - 2], lk, lsh[i - 2], lw[i - 2], lXg[i - 2]);
+        if (rho_s != 0) grho += lw[i - 2] * 2.0 * wv[i - 2] / rho_s;
+      }
+      for (int g = 0; g < 8; ++g) {
+        double vk = B1 * Ss[g], vy = lX7[g];
+        for (int jj = 2; jj <= 6; ++jj) {
+          vk += hh * dp_a(jj, 1) * lXg[jj - 2][g];
+          vy += lXg[jj - 2][g];
+        }
+        lkg[g] = vk;
+        lyg[g] = vy;
+      }
+    }
+  }
+  // k1_0 = f(y0) at the camera ray (colour state zero)
+  double y0[8];
+  camera_ray(A.cam, ray, A.x_pix, y0);
+  {
+    StageOut<double> o;
+    geo_rhs<KIND, double>(S, y0, o);
+    const StageSc ss = make_stage<KIND, double>(S, o);
+    double lsh0 = 0, lw0 = 0;
+    for (int c = 0; c < nc; ++c)
+      colour_stage_adjoint<KIND>(S, ss, A.colors[c], A.gcols, 0.0, lam[4 * c + 3], lam[4 * c + 2], lsh0, lw0, gT);
+    ga += geo_vjp<KIND>(S, y0, lkg, lsh0, lw0, lyg);
+    if (rho_s != 0) grho += lw0 * 2.0 * o.w / rho_s;
+  }
+  // camera: y0 = (pos, Mc v)
+  double v[4];
+  camera_local(A.cam, ray, A.x_pix, v);
+  double gth = lyg[2];
+  for (int a = 0; a < 4; ++a) {
+    double da = 0, dth = 0;
+    for (int b = 0; b < 4; ++b) {
+      da += A.dMa[a * 4 + b] * v[b];
+      dth += A.dMth[a * 4 + b] * v[b];
+    }
+    ga += lyg[4 + a] * da;
+    gth += lyg[4 + a] * dth;
+  }
+  G[0] = ga;
+  G[1] = gth;
+  G[2] = gT;
+  G[3] = grho;
+}
+
+}  // namespace rtw

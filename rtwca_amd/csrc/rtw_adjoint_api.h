@@ -1,0 +1,41 @@
+// rtw_adjoint_api.h -- arguments and launchers of the adjoint kernels (rtw_adjoint.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rtw_device.h"
+
+namespace rtw {
+
+enum { ADJ_EULER = 1 };  // ray took the Euler pole fallback: no gradient
+
+struct AdjArgs {
+  DevScene S;
+  DevOpts O;
+  DevCam cam;
+  const DevColor* colors;
+  const float2* gcols;
+  int x_pix, y_pix;
+  long long npix;
+  int K;                    // checkpoint stride in accepted steps
+  double* work;             // [npix][8 ncol]: (I, tau, kI, kT) per colour, then candidates
+  int32_t* info;            // [npix][4]: attempted, accepted, matter stage evaluations, flags
+  const int32_t* tape_info; // the counting pass's info of the recorded run (reverse, replay)
+  double* xyz;              // [npix][3]
+  const long long* off_h;   // per ray: first accepted-step slot in htape
+  const long long* off_c;   // per ray: first checkpoint slot in ckpt
+  double* htape;            // accepted step sizes
+  double* ckpt;             // [slot][2 n]: state y, then last slope k1
+  const double* dL;         // [npix][3]
+  double* seg;              // [chunk][K][2 n]: replayed states of the current segment
+  double* lam;              // [chunk][4 ncol]: adjoints of (I, tau) and of (kI, kT)
+  double* grad;             // [npix][4]: spin, inclination, temperature, density scale
+  double dMa[16], dMth[16]; // d(camera matrix)/d(spin), d/d(inclination)
+};
+
+// which = 0: counting forward, 1: recording forward, 2: reverse over rays
+// [ray0, ray0 + nrays), 3: replay of the recorded accepted steps over those rays.  Scene kinds other than the thick free-free disk are
+// rejected with hipErrorInvalidValue.
+hipError_t launch_adjoint(int which, int kind, const AdjArgs* dA, long long npix, long long ray0, long long nrays,
+                          hipStream_t st);
+
+}  // namespace rtw

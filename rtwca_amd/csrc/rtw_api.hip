@@ -12,9 +12,15 @@
 #include <vector>
 
 #include "../../include/rtw.h"
+#include "rtw_adjoint_api.h"
 #include "rtw_launch.h"
 
+#include <algorithm>
+#include <complex>
+
 using namespace rtw;
+
+static_assert(RTW_SCENE_GAUNT_FP64 == rtw::SCENE_GAUNT_FP64, "scene flag mismatch");
 
 // Camera pass of the wavefront schedule: 1 = persistent lane groups refilled
 // from the queue head, 0 = one ray per lane group in pixel order.
@@ -49,6 +55,12 @@ struct rtw_ctx {
   bool gaunt_loaded = false;
   std::vector<float> gaunt;  // 146 x 81 row-major
   DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs, pool;
+  DevBuf adj_work, adj_info, adj_tinfo, adj_offs, adj_tape, adj_ckpt, adj_seg, adj_lam, adj_dl, adj_grad;
+  // the recorded run of the last rtw_render_adjoint with a gradient (for rtw_adjoint_replay)
+  bool adj_recorded = false;
+  long long adj_npix = 0;
+  int adj_ncol = 0, adj_K = 0;
+  long long adj_chunk = 0;
   // event pairs around the integrator launches of the last render (kind 0 = lane
   // group, 1 = vacuum), read back by collect_stats
   std::vector<hipEvent_t> pev;
@@ -195,6 +207,45 @@ static DevCam make_cam(const DevScene& S, const rtw_camera& c) {
   return d;
 }
 
+// Derivatives of the Kerr camera matrix e R2 L R with respect to the spin and
+// the camera's theta, for the adjoint: only the inverse vierbein e depends on
+// them, and its derivatives are taken by complex step (e(p + i eps).imag / eps,
+// exact to rounding since every operation in e is analytic).
+template <class Z> static void inv_vierbein_kerr(Z a, Z r, Z th, double rs, Z e[4][4]) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) e[i][j] = Z(i == j ? 1.0 : 0.0);
+  Z s = std::sin(th), cth = std::cos(th);
+  Z Sg = r * r + (a * cth) * (a * cth), D = r * r - r * rs + a * a;
+  Z A = (r * r + a * a) * Sg + rs * r * (a * s) * (a * s);
+  e[0][0] = std::sqrt(A / (Sg * D));
+  e[3][0] = (rs * a * r / A) * e[0][0];
+  e[1][1] = std::sqrt(D / Sg);
+  e[2][2] = Z(1.0) / std::sqrt(Sg);
+  e[3][3] = std::sqrt(Sg / A) / s;
+}
+static void cam_derivs(const DevScene& S, const rtw_camera& c, double dMa[16], double dMth[16]) {
+  typedef std::complex<double> Z;
+  const double eps = 1e-30;
+  M4 R, L, R2, T, RR;
+  rot_intrinsic(c.point, R);
+  boost(-c.beta, L);
+  rot_final(c.dir, R2);
+  mmul(R2, L, T);
+  mmul(T, R, RR);  // the part of e R2 L R that does not depend on (spin, theta)
+  for (int which = 0; which < 2; ++which) {
+    Z e[4][4];
+    Z a(S.a, which == 0 ? eps : 0.0), th(c.pos[2], which == 1 ? eps : 0.0);
+    inv_vierbein_kerr<Z>(a, Z(c.pos[1]), th, S.rs, e);
+    double* out = which == 0 ? dMa : dMth;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        double v = 0;
+        for (int k = 0; k < 4; ++k) v += (e[i][k].imag() / eps) * RR[k][j];
+        out[i * 4 + j] = v;
+      }
+  }
+}
+
 // Scene constants (main/Kerr.source/source_geometry.jl:7-17 and the scene model files).
 static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o, DevScene& d) {
   std::memset(&d, 0, sizeof(d));
@@ -243,9 +294,16 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
     float loggam = (float)std::log10(gam2);
     float frac_j = (loggam + 6.0f) / 0.2f;
     int fj = (int)std::floor(frac_j), cj = (int)std::ceil(frac_j + 1e-3f);
-    d.g_ok_j = (fj >= 1 && cj <= 81) ? 1 : 0;
     d.gw0 = (float)cj - frac_j;
     d.gw1 = frac_j - (float)fj;
+    if (s.flags & RTW_SCENE_GAUNT_FP64) {  // smooth form: FP64 coordinate, adjacent columns
+      const double fjd = (std::log10(gam2) + 6.0) / 0.2;
+      fj = (int)std::floor(fjd);
+      cj = fj + 1;
+      d.gw0d = cj - fjd;
+      d.gw1d = fjd - fj;
+    }
+    d.g_ok_j = (fj >= 1 && cj <= 81) ? 1 : 0;
     std::vector<float2> cols(146);
     for (int i = 0; i < 146; ++i) {
       cols[i].x = d.g_ok_j ? ctx->gaunt[i * 81 + (fj - 1)] : 1.0f;
@@ -661,6 +719,59 @@ int rtw_render_dev(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, i
                         nullptr, false);
 }
 
+int rtw_adjoint_replay(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cam, const rtw_integrator* opts,
+                       const double* lam, int32_t n_lambda, double* out_xyz, int32_t* out_info) {
+  if (!c) return RTW_E_ARG;
+  if (!scene || !cam || !opts || !lam || !out_xyz) return fail(c, RTW_E_ARG, "NULL or invalid argument");
+  if (!c->adj_recorded) return fail(c, RTW_E_ARG, "no recorded adjoint run on this context");
+  if (scene->kind != RTW_SCENE_KERR_THICK_FF) return fail(c, RTW_E_UNSUPPORTED, "thick free-free disk only");
+  if (cam_is_singular(cam->pos) || !(std::fabs(cam->beta) < 1)) return fail(c, RTW_E_CAMERA, "invalid camera");
+  const int ny = cam_y_pix(*cam);
+  const long long npix = (long long)cam->x_pix * ny;
+  if (npix != c->adj_npix || n_lambda != c->adj_ncol)
+    return fail(c, RTW_E_ARG, "camera grid or colours differ from the recorded run");
+  for (int k = 0; k < n_lambda; ++k)
+    if (!(lam[k] > 0)) return fail(c, RTW_E_ARG, "wavelengths must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  DevScene S;
+  int rc = make_scene(c, *scene, *opts, S);
+  if (rc) return rc;
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.S = S;
+  A.O.rtol = opts->rtol;
+  A.O.atol = opts->atol;
+  A.O.max_steps = opts->max_steps;
+  A.O.tau_thr = -std::log(opts->atol);
+  A.O.ncol = n_lambda;
+  A.O.n = 8 + 2 * n_lambda;
+  A.cam = make_cam(S, *cam);
+  A.x_pix = cam->x_pix;
+  A.y_pix = ny;
+  A.npix = npix;
+  A.K = c->adj_K;
+  std::vector<DevColor> cols = make_colors(S, lam, n_lambda);
+  HIPCHK(c, hipMemcpyAsync(c->colors.p, cols.data(), sizeof(DevColor) * n_lambda, hipMemcpyHostToDevice, st));
+  A.colors = c->colors.as<DevColor>();
+  A.gcols = c->gcols.as<float2>();
+  A.info = c->adj_info.as<int32_t>();
+  A.tape_info = c->adj_tinfo.as<int32_t>();
+  A.xyz = c->xyz.as<double>();
+  A.off_h = c->adj_offs.as<long long>();
+  A.off_c = A.off_h + npix;
+  A.htape = c->adj_tape.as<double>();
+  A.seg = c->adj_seg.as<double>();
+  HIPCHK(c, hipMemcpyAsync(c->kargs.p, &A, sizeof(AdjArgs), hipMemcpyHostToDevice, st));
+  const AdjArgs* dA = c->kargs.as<AdjArgs>();
+  for (long long r0 = 0; r0 < npix; r0 += c->adj_chunk)
+    HIPCHK(c, launch_adjoint(3, S.kind, dA, npix, r0, std::min(c->adj_chunk, npix - r0), st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipMemcpy(out_xyz, A.xyz, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost));
+  if (out_info) HIPCHK(c, hipMemcpy(out_info, A.info, sizeof(int32_t) * 4 * npix, hipMemcpyDeviceToHost));
+  return RTW_OK;
+}
+
 int rtw_last_stats(rtw_ctx* c, rtw_stats* stats) {
   if (!c) return RTW_E_ARG;
   if (!stats) return fail(c, RTW_E_ARG, "NULL stats");
@@ -690,9 +801,159 @@ int rtw_xyz_to_srgb16(rtw_ctx* c, const double* xyz, int32_t x_pix, int32_t y_pi
   return RTW_OK;
 }
 
-int rtw_render_adjoint(rtw_ctx* c, const rtw_scene*, const rtw_camera*, const rtw_integrator*, const double*, int32_t,
-                       const double*, double*, double*, rtw_stats*) {
-  return fail(c, RTW_E_UNSUPPORTED, "adjoint not built");
+int rtw_render_adjoint(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cam, const rtw_integrator* opts,
+                       const double* lam, int32_t n_lambda, const double* dL_dxyz, double* out_xyz, double* out_grad,
+                       double* out_grad_pix, int32_t* out_info, rtw_stats* stats) {
+  if (!c) return RTW_E_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!scene || !cam || !opts || !lam || n_lambda < 1) return fail(c, RTW_E_ARG, "NULL or invalid argument");
+  if (dL_dxyz && !out_grad && !out_grad_pix) return fail(c, RTW_E_ARG, "dL_dxyz given without a gradient output");
+  if (scene->kind != RTW_SCENE_KERR_THICK_FF)
+    return fail(c, RTW_E_UNSUPPORTED, "the adjoint covers the thick free-free disk scene");
+  if (scene->flags & RTW_SCENE_REF_F32) return fail(c, RTW_E_UNSUPPORTED, "the adjoint runs in FP64 only");
+  if (n_lambda > 64) return fail(c, RTW_E_UNSUPPORTED, "at most 64 wavelengths");
+  if (!(opts->rtol > 0) || !(opts->atol > 0) || !(opts->max_dt_scale > 0) || !(opts->max_steps >= 1))
+    return fail(c, RTW_E_ARG, "invalid tolerances/max_dt_scale/max_steps");
+  for (int k = 0; k < n_lambda; ++k)
+    if (!(lam[k] > 0)) return fail(c, RTW_E_ARG, "wavelengths must be positive");
+  if (cam->x_pix < 1 || !(cam->fov_hor > 0 && cam->fov_hor < 180) || !(cam->fov_vert > 0 && cam->fov_vert < 180))
+    return fail(c, RTW_E_ARG, "invalid camera grid");
+  if (cam_is_singular(cam->pos)) return fail(c, RTW_E_CAMERA, "camera on a coordinate singularity");
+  if (!(std::fabs(cam->beta) < 1)) return fail(c, RTW_E_ARG, "|beta| must be < 1");
+  if (!(scene->density > 0)) return fail(c, RTW_E_ARG, "density scale must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  DevScene S;
+  int rc = make_scene(c, *scene, *opts, S);
+  if (rc) return rc;
+  const int ny = cam_y_pix(*cam);
+  if (ny < 1) return fail(c, RTW_E_ARG, "camera has no rows");
+  const long long npix = (long long)cam->x_pix * ny;
+  if (npix >= (1LL << 31)) return fail(c, RTW_E_ARG, "too many rays in one call");
+  const int nc = n_lambda, n = 8 + 2 * n_lambda;
+
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.S = S;
+  A.O.rtol = opts->rtol;
+  A.O.atol = opts->atol;
+  A.O.max_steps = opts->max_steps;
+  A.O.tau_thr = -std::log(opts->atol);
+  A.O.fp64_radius = 0;
+  A.O.ncol = nc;
+  A.O.n = n;
+  A.cam = make_cam(S, *cam);
+  cam_derivs(S, *cam, A.dMa, A.dMth);
+  A.x_pix = cam->x_pix;
+  A.y_pix = ny;
+  A.npix = npix;
+  std::vector<DevColor> cols = make_colors(S, lam, n_lambda);
+  HIPCHK(c, c->colors.ensure(sizeof(DevColor) * (n_lambda + 1)));
+  HIPCHK(c, hipMemcpyAsync(c->colors.p, cols.data(), sizeof(DevColor) * n_lambda, hipMemcpyHostToDevice, st));
+  A.colors = c->colors.as<DevColor>();
+  A.gcols = c->gcols.as<float2>();
+  HIPCHK(c, c->adj_work.ensure(sizeof(double) * 8 * nc * npix));
+  HIPCHK(c, c->adj_info.ensure(sizeof(int32_t) * 4 * npix));
+  HIPCHK(c, c->xyz.ensure(sizeof(double) * 3 * npix));
+  A.work = c->adj_work.as<double>();
+  A.info = c->adj_info.as<int32_t>();
+  A.xyz = c->xyz.as<double>();
+  A.K = 1;
+  HIPCHK(c, c->kargs.ensure(sizeof(AdjArgs)));
+  const AdjArgs* dA = c->kargs.as<AdjArgs>();
+  auto upload = [&]() -> hipError_t { return hipMemcpyAsync(c->kargs.p, &A, sizeof(AdjArgs), hipMemcpyHostToDevice, st); };
+
+  // 1. counting forward
+  HIPCHK(c, upload());
+  HIPCHK(c, hipEventRecord(c->ev0, st));
+  HIPCHK(c, launch_adjoint(0, S.kind, dA, npix, 0, npix, st));
+  std::vector<int32_t> info(4 * npix);
+  HIPCHK(c, hipMemcpyAsync(info.data(), A.info, sizeof(int32_t) * 4 * npix, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  long long sum_acc = 0, sum_att = 0;
+  for (long long r = 0; r < npix; ++r) {
+    sum_acc += info[4 * r + 1];
+    sum_att += info[4 * r];
+  }
+  if (out_xyz) HIPCHK(c, hipMemcpy(out_xyz, A.xyz, sizeof(double) * 3 * npix, hipMemcpyDeviceToHost));
+  if (out_info) std::memcpy(out_info, info.data(), sizeof(int32_t) * 4 * npix);
+  std::vector<double> gp;
+  if (dL_dxyz) {
+    // 2. tape layout: checkpoint stride ~ sqrt(mean accepted steps), memory of
+    //    checkpoints + one chunk of segment buffers balanced
+    const double mean_acc = (double)sum_acc / (double)npix;
+    const int K = std::max(8, std::min(128, (int)std::ceil(std::sqrt(mean_acc + 1.0))));
+    A.K = K;
+    std::vector<long long> offh(npix), offc(npix);
+    long long ph = 0, pc = 0;
+    for (long long r = 0; r < npix; ++r) {
+      offh[r] = ph;
+      offc[r] = pc;
+      ph += info[4 * r + 1];
+      pc += info[4 * r + 1] / K + 1;
+    }
+    const long long chunk = std::min<long long>(npix, 1 << 16);
+    HIPCHK(c, c->adj_offs.ensure(sizeof(long long) * 2 * npix));
+    HIPCHK(c, c->adj_tape.ensure(sizeof(double) * (ph + 1)));
+    HIPCHK(c, c->adj_ckpt.ensure(sizeof(double) * 2 * n * pc));
+    HIPCHK(c, c->adj_seg.ensure(sizeof(double) * (size_t)K * 2 * n * chunk));
+    HIPCHK(c, c->adj_lam.ensure(sizeof(double) * 4 * nc * chunk));
+    HIPCHK(c, c->adj_dl.ensure(sizeof(double) * 3 * npix));
+    HIPCHK(c, c->adj_grad.ensure(sizeof(double) * 4 * npix));
+    long long* doffs = c->adj_offs.as<long long>();
+    HIPCHK(c, hipMemcpyAsync(doffs, offh.data(), sizeof(long long) * npix, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(doffs + npix, offc.data(), sizeof(long long) * npix, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->adj_dl.p, dL_dxyz, sizeof(double) * 3 * npix, hipMemcpyHostToDevice, st));
+    HIPCHK(c, c->adj_tinfo.ensure(sizeof(int32_t) * 4 * npix));
+    HIPCHK(c, hipMemcpyAsync(c->adj_tinfo.p, A.info, sizeof(int32_t) * 4 * npix, hipMemcpyDeviceToDevice, st));
+    A.tape_info = c->adj_tinfo.as<int32_t>();
+    A.off_h = doffs;
+    A.off_c = doffs + npix;
+    A.htape = c->adj_tape.as<double>();
+    A.ckpt = c->adj_ckpt.as<double>();
+    A.dL = c->adj_dl.as<double>();
+    A.seg = c->adj_seg.as<double>();
+    A.lam = c->adj_lam.as<double>();
+    A.grad = c->adj_grad.as<double>();
+    HIPCHK(c, upload());
+    // 3. recording forward (the same arithmetic, so the same accepted steps)
+    HIPCHK(c, launch_adjoint(1, S.kind, dA, npix, 0, npix, st));
+    // 4. reverse, in chunks of rays sharing the segment buffers
+    for (long long r0 = 0; r0 < npix; r0 += chunk)
+      HIPCHK(c, launch_adjoint(2, S.kind, dA, npix, r0, std::min(chunk, npix - r0), st));
+    gp.resize(4 * npix);
+    HIPCHK(c, hipMemcpyAsync(gp.data(), A.grad, sizeof(double) * 4 * npix, hipMemcpyDeviceToHost, st));
+    c->adj_recorded = true;
+    c->adj_npix = npix;
+    c->adj_ncol = nc;
+    c->adj_K = K;
+    c->adj_chunk = chunk;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (dL_dxyz) {
+    double g[4] = {0, 0, 0, 0};
+    for (long long r = 0; r < npix; ++r)
+      for (int k = 0; k < 4; ++k) g[k] += gp[4 * r + k];
+    if (out_grad) std::memcpy(out_grad, g, sizeof(g));
+    if (out_grad_pix) std::memcpy(out_grad_pix, gp.data(), sizeof(double) * 4 * npix);
+  }
+  c->nlaunch = 0;
+  c->last_stream = st;
+  c->last_rays = npix;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->rays = npix;
+    stats->steps_attempted = sum_att;
+    stats->steps_accepted = sum_acc;
+    stats->steps_fp64 = sum_att;
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->kernel_ms = ms;
+    stats->lanes_per_ray = 1;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return RTW_OK;
 }
 
 }  // extern "C"

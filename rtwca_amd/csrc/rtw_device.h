@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace rtw {
 
 constexpr double PI = 3.141592653589793;
@@ -26,6 +28,7 @@ constexpr double FF_T0 = 1e4;           // thick disk opacity normalisation temp
 constexpr double FF_NU0 = 1.0 / 550.0;  // and frequency (nm^-1)
 
 enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5 };
+constexpr int SCENE_GAUNT_FP64 = 2;  // Gaunt interpolation in FP64 (scene flag)
 enum { CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
 
 // Scene constants, derived on the host from rtw_scene.
@@ -44,6 +47,7 @@ struct DevScene {
   double ff_C;         // kappa * sqrt(T0 / T)
   float gw0, gw1;      // Gaunt column weights (ceil_j - frac_j), (frac_j - floor_j)
   int g_ok_j;          // Gaunt column indices in range
+  double gw0d, gw1d;   // the same weights in FP64 (SCENE_GAUNT_FP64)
   int ref_f32;         // Float32 Christoffel symbols
 };
 
@@ -70,6 +74,39 @@ struct DevCam {
 };
 
 template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
+
+// Local direction of pixel pix of a pose (initialize_camera, initialization.jl:213-240).
+__device__ __forceinline__ void camera_local(const DevCam& C, long long pix, int x_pix, double v[4]) {
+  const int i = (int)(pix % x_pix), j = (int)(pix / x_pix);
+  const double w = C.rho_h * ((i + 1) - 0.5) - C.W;
+  const double hh = C.rho_v * ((j + 1) - 0.5) - C.H;
+  const double th = atan(sqrt(w * w + hh * hh));
+  double phi;
+  if (w > 0) phi = atan(hh / w);
+  else if (w < 0 && hh >= 0) phi = atan(hh / w) + PI;
+  else if (w < 0 && hh < 0) phi = atan(hh / w) - PI;
+  else if (w == 0 && hh > 0) phi = PI / 2;
+  else if (w == 0 && hh < 0) phi = -PI / 2;
+  else phi = 0;
+  double st, ct, sp, cp;
+  sincos(th, &st, &ct);
+  sincos(phi, &sp, &cp);
+  v[0] = 1.0; v[1] = -st * cp; v[2] = -st * sp; v[3] = -ct;
+}
+// Camera ray (initialization.jl:327): position of the pose, momentum e R2 L R v.
+__device__ __forceinline__ void camera_ray(const DevCam& C, long long pix, int x_pix, double R[8]) {
+  double v[4];
+  camera_local(C, pix, x_pix, v);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) R[k] = C.pos[k];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double s = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) s += C.Mc[a * 4 + b] * v[b];
+    R[4 + a] = s;
+  }
+}
 
 __device__ __forceinline__ void sc_(double x, double& s, double& c) { sincos(x, &s, &c); }
 __device__ __forceinline__ void sc_(float x, float& s, float& c) { sincosf(x, &s, &c); }
@@ -175,6 +212,9 @@ __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4
   acc[3] = T(-2) * (T(Gp_tr) * tr + T(Gp_tth) * tth + T(Gp_rph) * rph + T(Gp_thph) * thph);
 }
 
+template <int KIND, class T>
+__device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T x[8], StageOut<T>& o);
+
 // Geodesic right-hand side and stage scalars at x (calc_ray_derivative!,
 // main/Iterators/LttM/integrators_LttM.jl:11-36).
 template <int KIND, class T>
@@ -227,21 +267,35 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
       o.sh = -(ut * (gtt * kt));
     }
   } else {
-    // Kerr.source/source_geometry.jl:55-75 (metric), :195-243 (symbols)
+    kerr_rhs<KIND, T>(S, T(S.a), x, o);
+  }
+}
+
+// Kerr family (Kerr.source/source_geometry.jl:55-75 metric, :195-243 symbols;
+// source_model.jl:109-137 velocity) with the spin as an argument of type T, so
+// the adjoint can carry its derivative.
+template <int KIND, class T>
+__device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T x[8], StageOut<T>& o) {
+  constexpr bool F = fast_math_v<T>;
+  const T r = x[1], th = x[2];
+  const T kt = x[4], kr = x[5], kth = x[6], kph = x[7];
+  o.d[0] = kt; o.d[1] = kr; o.d[2] = kth; o.d[3] = kph;
+  o.w = T(0);
+  {
     const T k4[4] = {kt, kr, kth, kph};
     T acc[4];
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (std::is_same<T, double>::value) {
       if (S.ref_f32)
         kerr_accel<float, T>((float)r, (float)th, (float)S.a, (float)S.rs, k4, acc);
       else
-        kerr_accel<T, T>(r, th, T(S.a), T(S.rs), k4, acc);
+        kerr_accel<T, T>(r, th, a, T(S.rs), k4, acc);
     } else {
-      kerr_accel<T, T, F>(r, th, T(S.a), T(S.rs), k4, acc);
+      kerr_accel<T, T, F>(r, th, a, T(S.rs), k4, acc);
     }
     o.d[4] = acc[0]; o.d[5] = acc[1]; o.d[6] = acc[2]; o.d[7] = acc[3];
     T s, c;
     sc_(th, s, c);
-    const T a = T(S.a), rs = T(S.rs), M = T(S.M);
+    const T rs = T(S.rs), M = T(S.M);
     T r2 = r * r, a2 = a * a, s2 = s * s;
     T Sg = r2 + a2 * c * c;
     T D = r2 - r * rs + a2;
@@ -349,6 +403,32 @@ template <int KIND> __device__ __forceinline__ void keepinbounds(double* x) {
   }
 }
 
+// keepinbounds! that also reports the parity of its theta reflections (each one
+// negates theta and k^theta; the phi shifts by pi have unit derivative).
+template <int KIND> __device__ __forceinline__ int keepinbounds_parity(double* x) {
+  int flips = 0;
+  if (KIND == MINK && x[1] <= 0) {
+    x[1] = -x[1] + NO_DIV_ZERO;
+    x[5] = -x[5];
+    x[2] -= PI;
+    x[6] = -x[6];
+  }
+  if (x[2] < 0) {
+    x[2] = -x[2];
+    x[3] -= PI;
+    x[6] = -x[6];
+    flips++;
+  }
+  x[2] = jmod(x[2], 2 * PI);
+  if (x[2] >= PI) {
+    x[2] = 2 * PI - x[2];
+    x[3] += PI;
+    x[6] = -x[6];
+    flips++;
+  }
+  return flips & 1;
+}
+
 // van Hoof Gaunt lookup (main/emission_functions.jl:54-78) with the row index from
 // log10(u) and the two columns for the scene temperature staged in LDS as
 // float2 (col floor_j, col ceil_j) per row.  Float32 as the reference, written
@@ -375,6 +455,57 @@ __device__ __forceinline__ float gaunt_row_lookup(float logu, const float2* col,
     return __fadd_rn(__fmul_rn(w0, g1), __fmul_rn(w1, g2));
   }
   return 1.2f;
+}
+
+// Smooth form of the two-column lookup (SCENE_GAUNT_FP64): the same table and
+// one-cell index offset, in FP64, with plain bilinear weights (ceil = floor + 1;
+// the Float32 form's ceil(frac + 1e-3) jumps to weights summing to 2 on the last
+// 0.001 of each cell).
+__device__ __forceinline__ double gaunt_row_lookup_f64(double logu, const float2* col, double w0, double w1, int ok_j) {
+  const double frac_i = (logu + 16.0) / 0.2;
+  if (!(fabs(frac_i) < 1e9)) return 1.2;
+  const int fi = (int)floor(frac_i), ci = fi + 1;
+  if (ok_j && fi >= 1 && ci <= 146) {
+    const float2 lo = col[fi - 1], hi = col[ci - 1];
+    const double a = ci - frac_i, b = frac_i - fi;
+    const double g1 = a * lo.x + b * hi.x, g2 = a * lo.y + b * hi.y;
+    return w0 * g1 + w1 * g2;
+  }
+  return 1.2;
+}
+__device__ __forceinline__ void gaunt_slopes_f64(double logu, const float2* col, double w0, double w1, int ok_j,
+                                                 double& dg_dlogu, double& dg_dlgam) {
+  const double frac_i = (logu + 16.0) / 0.2;
+  dg_dlogu = 0;
+  dg_dlgam = 0;
+  if (!(fabs(frac_i) < 1e9)) return;
+  const int fi = (int)floor(frac_i), ci = fi + 1;
+  if (ok_j && fi >= 1 && ci <= 146) {
+    const float2 lo = col[fi - 1], hi = col[ci - 1];
+    const double a = ci - frac_i, b = frac_i - fi;
+    const double g1 = a * lo.x + b * hi.x, g2 = a * lo.y + b * hi.y;
+    dg_dlogu = 5.0 * (w0 * ((double)hi.x - lo.x) + w1 * ((double)hi.y - lo.y));
+    dg_dlgam = 5.0 * (g2 - g1);
+  }
+}
+
+// Slopes of the two-column lookup above (piecewise constant): dg/dlog10(u) along
+// the rows and dg/dlog10(gamma^2) across the column pair (frac = (log + 16)/0.2
+// and (log + 6)/0.2, so both scale by 5).  Zero where the lookup returns 1.2.
+__device__ __forceinline__ void gaunt_slopes(float logu, const float2* col, float w0, float w1, int ok_j,
+                                             double& dg_dlogu, double& dg_dlgam) {
+  const float frac_i = gaunt_row_coord(logu);
+  const int fi = (int)floorf(frac_i);
+  const int ci = (int)ceilf(__fadd_rn(frac_i, 1e-3f));
+  dg_dlogu = 0;
+  dg_dlgam = 0;
+  if (ok_j && fi >= 1 && ci <= 146) {
+    const float2 lo = col[fi - 1], hi = col[ci - 1];
+    const double a = (double)ci - frac_i, b = (double)frac_i - fi;
+    const double g1 = a * lo.x + b * hi.x, g2 = a * lo.y + b * hi.y;
+    dg_dlogu = 5.0 * ((double)w0 * ((double)hi.x - lo.x) + (double)w1 * ((double)hi.y - lo.y));
+    dg_dlgam = 5.0 * (g2 - g1);
+  }
 }
 
 // The same lookup for FP32 stages, on the single column gc[k] = w0 col[k].x +

@@ -1,0 +1,22 @@
+// Adjoint kernels of the thick free-free disk scene.
+#include "rtw_adjoint.h"
+
+namespace rtw {
+
+hipError_t launch_adjoint(int which, int kind, const AdjArgs* dA, long long npix, long long ray0, long long nrays,
+                          hipStream_t st) {
+  if (kind != KERR_FF) return hipErrorInvalidValue;
+  constexpr int B = 64;
+  if (which == 0) {
+    hipLaunchKernelGGL((adj_forward<KERR_FF, false>), dim3((unsigned)((npix + B - 1) / B)), dim3(B), 0, st, dA);
+  } else if (which == 1) {
+    hipLaunchKernelGGL((adj_forward<KERR_FF, true>), dim3((unsigned)((npix + B - 1) / B)), dim3(B), 0, st, dA);
+  } else if (which == 2) {
+    hipLaunchKernelGGL((adj_reverse<KERR_FF>), dim3((unsigned)((nrays + B - 1) / B)), dim3(B), 0, st, dA, ray0, nrays);
+  } else {
+    hipLaunchKernelGGL((adj_replay<KERR_FF>), dim3((unsigned)((nrays + B - 1) / B)), dim3(B), 0, st, dA, ray0, nrays);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rtw

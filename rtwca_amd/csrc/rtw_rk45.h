@@ -194,8 +194,12 @@ __device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, 
     // a = C rho^2 (nu0/nu)^3 (1 - e^-u) g_ff,  j = a B_nu(T)  (Kirchhoff)
     T u = T(c.uf) * T(ss.sh);
     T em = exp_(-u);
-    float g = gaunt_row_lookup((float)(c.lc + ss.lg), gcol, S.gw0, S.gw1, S.g_ok_j);
-    T base = T(ss.P) * T(c.if3) * T(g);
+    T g;
+    if (sizeof(T) == 8 && (S.flags & SCENE_GAUNT_FP64))
+      g = T(gaunt_row_lookup_f64(c.lc + ss.lg, gcol, S.gw0d, S.gw1d, S.g_ok_j));
+    else
+      g = T(gaunt_row_lookup((float)(c.lc + ss.lg), gcol, S.gw0, S.gw1, S.g_ok_j));
+    T base = T(ss.P) * T(c.if3) * g;
     kT = (double)(base * (T(1) - em));
     kI = (double)(base * ES * exp_(-u - T(tau)));
   } else {
@@ -319,32 +323,7 @@ struct Integrator {
 #pragma unroll
       for (int k = 0; k < 8; ++k) R[k] = A.init_rays[pix + k * A.npix];
     } else {
-      // initialize_camera (initialization.jl:213-240, 327): pixel direction, then e R2 L R
-      const DevCam& C = A.cams[pose];
-      int i = (int)(pix % A.x_pix), j = (int)(pix / A.x_pix);
-      double w = C.rho_h * ((i + 1) - 0.5) - C.W;
-      double hh = C.rho_v * ((j + 1) - 0.5) - C.H;
-      double th = atan(sqrt(w * w + hh * hh));
-      double phi;
-      if (w > 0) phi = atan(hh / w);
-      else if (w < 0 && hh >= 0) phi = atan(hh / w) + PI;
-      else if (w < 0 && hh < 0) phi = atan(hh / w) - PI;
-      else if (w == 0 && hh > 0) phi = PI / 2;
-      else if (w == 0 && hh < 0) phi = -PI / 2;
-      else phi = 0;
-      double st, ct, sp, cp;
-      sincos(th, &st, &ct);
-      sincos(phi, &sp, &cp);
-      double v[4] = {1.0, -st * cp, -st * sp, -ct};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) R[k] = C.pos[k];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        double s = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) s += C.Mc[a * 4 + b] * v[b];
-        R[4 + a] = s;
-      }
+      camera_ray(A.cams[pose], pix, A.x_pix, R);
     }
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) { sI[m] = 0; sT[m] = 0; lI[m] = 0; lT[m] = 0; }

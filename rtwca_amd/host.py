@@ -47,7 +47,8 @@ def _load():
     L.rtw_render_dev.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P]
     L.rtw_xyz_to_srgb16.argtypes = [P, P, C.c_int32, C.c_int32, P]
     L.rtw_last_stats.argtypes = [P, P]
-    L.rtw_render_adjoint.argtypes = [P, P, P, P, P, C.c_int32, P, P, P, P]
+    L.rtw_render_adjoint.argtypes = [P, P, P, P, P, C.c_int32, P, P, P, P, P, P]
+    L.rtw_adjoint_replay.argtypes = [P, P, P, P, P, C.c_int32, P, P]
     _lib = L
     return L
 
@@ -122,6 +123,38 @@ class Context:
                 "rays": None if rays is None else rays.reshape(n, npix).T.copy(),
                 "stats": {k: getattr(st, k) for k, _ in abi.Stats._fields_ if k != "reserved"},
                 "x_pix": c0.x_pix, "y_pix": ny}
+
+    def render_adjoint(self, scene, camera, integ, lam, dL_dxyz=None, want_grad_pix=False):
+        """rtw_render_adjoint: forward image, per-pixel info (attempted, accepted, matter stage
+        evaluations, flags) and, with dL_dxyz, dL/d(spin, inclination, T, density scale)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        W = np.tan(np.pi * camera.fov_hor / 180 / 2)
+        H = np.tan(np.pi * camera.fov_vert / 180 / 2)
+        npix = camera.x_pix * int(np.rint(camera.x_pix * H / W))
+        xyz = np.zeros((npix, 3))
+        info = np.zeros((npix, 4), dtype=np.int32)
+        dl = None if dL_dxyz is None else np.ascontiguousarray(dL_dxyz, dtype=np.float64).reshape(npix, 3)
+        grad = np.zeros(4) if dl is not None else None
+        gpix = np.zeros((npix, 4)) if (dl is not None and want_grad_pix) else None
+        st = abi.Stats()
+        rc = self.L.rtw_render_adjoint(self.h, _ref(scene), _ref(camera), _ref(integ), _ptr(lam), len(lam), _ptr(dl),
+                                       _ptr(xyz), _ptr(grad), _ptr(gpix), _ptr(info), _ref(st))
+        self._check(rc, "rtw_render_adjoint")
+        return {"xyz": xyz, "info": info, "grad": grad, "grad_pix": gpix,
+                "stats": {k: getattr(st, k) for k, _ in abi.Stats._fields_ if k != "reserved"}}
+
+    def adjoint_replay(self, scene, camera, integ, lam):
+        """rtw_adjoint_replay: the last recorded accepted-step map at new parameters."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        W = np.tan(np.pi * camera.fov_hor / 180 / 2)
+        H = np.tan(np.pi * camera.fov_vert / 180 / 2)
+        npix = camera.x_pix * int(np.rint(camera.x_pix * H / W))
+        xyz = np.zeros((npix, 3))
+        info = np.zeros((npix, 4), dtype=np.int32)
+        rc = self.L.rtw_adjoint_replay(self.h, _ref(scene), _ref(camera), _ref(integ), _ptr(lam), len(lam),
+                                       _ptr(xyz), _ptr(info))
+        self._check(rc, "rtw_adjoint_replay")
+        return {"xyz": xyz, "info": info}
 
     def last_stats(self):
         """rtw_last_stats: counters and per-kernel device times of the last render (waits for it)."""

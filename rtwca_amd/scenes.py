@@ -121,7 +121,7 @@ def kerr_disk(spin=0.998, ref_f32=True):
 
 
 def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, r_out=12.0,
-                  h_over_r=0.1, z_cut=4.0):
+                  h_over_r=0.1, z_cut=4.0, gaunt_fp64=False):
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Gaussian thick
     disk rho = rho_s exp(-z^2 / 2H^2), H = 0.1 r, |z| <= 4H, radiating free-free with the
     van Hoof Gaunt factor (main/emission_functions.jl:54-78) and Kirchhoff's law."""
@@ -134,6 +134,8 @@ def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, 
     s.r_out = r_out
     s.h_over_r = h_over_r
     s.z_cut = z_cut
+    if gaunt_fp64:
+        s.flags |= abi.SCENE_GAUNT_FP64
     return s
 
 
@@ -185,7 +187,7 @@ def config(idx, x_pix=None, precision=abi.PREC_FP64, schedule=abi.SCHED_PERSISTE
         it = abi.make_integrator(1e-6, 1e-6, 0.1, 1e4, precision, schedule, 3.0)
         return Workload("kerr_a0.9_thick_ff_1024", sc, [cam], it, lam)
     if idx == 4:
-        sc = kerr_thick_ff(spin=0.9, r_in=kerr_isco_ref(0.9))
+        sc = kerr_thick_ff(spin=0.9, r_in=kerr_isco_ref(0.9), gaunt_fp64=True)
         cam = abi.make_camera([0, 10, 60 * PI / 180, 0], [0, 0], 0.0, [-PI / 2, 0, 0],
                               x_pix or 256, 60, 60)
         it = abi.make_integrator(1e-8, 1e-8, 0.1, 1e4, precision, schedule)

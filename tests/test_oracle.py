@@ -342,6 +342,37 @@ def test_gaunt_bilinear_midpoint_and_edges():
     assert O.gaunt(1e30, 1.0) == pytest.approx(1.2)
 
 
+def test_gaunt_fp64_nodes_midpoints_and_float_agreement():
+    """The smooth form (scene flag GAUNT_FP64) keeps the table and the one-cell
+    index offset (Q7) but uses plain bilinear weights, so it is continuous where
+    the reference's weights jump to a sum of 2 (Q8); elsewhere it differs from the
+    Float32 form only by that form's rounding."""
+    t = _table().astype(float)
+    for n, m in ((5, 7), (40, 20), (82, 36), (120, 70)):
+        g = O.gaunt_f64(10 ** (-6 + 0.2 * m), 10 ** (-16 + 0.2 * n))
+        assert g == pytest.approx(t[n - 1, m - 1], rel=1e-9), (n, m)
+    n, m = 60, 30
+    g = O.gaunt_f64(10 ** (-6 + 0.2 * (m + 0.5)), 10 ** (-16 + 0.2 * (n + 0.5)))
+    assert g == pytest.approx(0.25 * (t[n - 1, m - 1] + t[n, m - 1] + t[n - 1, m] + t[n, m]), rel=1e-9)
+    fr = 0.9995
+    g = O.gaunt_f64(10 ** (-6 + 0.2 * m), 10 ** (-16 + 0.2 * (n + fr)))
+    assert g == pytest.approx((1 - fr) * t[n - 1, m - 1] + fr * t[n, m - 1], rel=1e-9)
+    # continuous across the cell edge where the Float32 form jumps
+    below = O.gaunt_f64(10 ** (-6 + 0.2 * m), 10 ** (-16 + 0.2 * (n + 1 - 1e-9)))
+    above = O.gaunt_f64(10 ** (-6 + 0.2 * m), 10 ** (-16 + 0.2 * (n + 1 + 1e-9)))
+    assert abs(below - above) < 1e-7 * abs(above)
+    assert O.gaunt_f64(10.0, 1e-20) == 1.2 and O.gaunt_f64(1e30, 1.0) == 1.2
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        gam2, u = 10 ** rng.uniform(-5, 9), 10 ** rng.uniform(-15, 13)
+        a, b = O.gaunt(gam2, u), O.gaunt_f64(gam2, u)
+        frac = ((np.log10(u) + 16) / 0.2) % 1
+        fracj = ((np.log10(gam2) + 6) / 0.2) % 1
+        if frac > 0.998 or fracj > 0.998:  # the Float32 form's weights sum to 2 there
+            continue
+        assert b == pytest.approx(a, rel=2e-5)
+
+
 # ---------------------------------------------------------------- camera
 @pytest.mark.parametrize("idx", [1, 2, 3, 4])
 def test_camera_rays_are_null(idx):
