@@ -1,6 +1,11 @@
 """Benchmark: rays/s and RK45 steps/s of the Kerr a=0.9 thick-disk 1024x1024 render.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--x-pix 1024] [--no-cpu]
+    python bench.py --workload synch [--poses 64] [--x-pix 512]   # batched synchrotron scene
+
+The second form times the 64-pose thermal-synchrotron torus + jet batch (64 cameras
+along an inclined orbit at r = 50, 512x512 each, one render call, the same
+wavefront schedule) and prints one JSON line of the same shape for it.
 
 One "step" of this benchmark is one full render of the workload (every ray of the
 1024x1024 camera integrated to termination with 64 wavelengths) by the
@@ -37,7 +42,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32 (AMD published)
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_1024_mixed_hbm.md")
+TRAFFIC_SUMMARY = {3: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_1024_mixed_hbm.md"),
+                   5: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_synch_64x512_mixed_hbm.md")}
+WORKLOADS = {"kerr": 3, "synch": 5}
 
 
 def parse():
@@ -45,7 +52,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--x-pix", type=int, default=1024)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kerr")
+    ap.add_argument("--x-pix", type=int, default=None, help="default 1024 (kerr), 512 (synch)")
+    ap.add_argument("--poses", type=int, default=64, help="synch: number of the 64 orbit poses to render")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=32768)
@@ -60,23 +69,26 @@ def parse():
     return ap.parse_args()
 
 
-def flops_table():
+def flops_table(idx=3):
     with open(os.path.join(ROOT, "tests", "golden", "flops_per_step.json")) as f:
-        return json.load(f)["cfg3"]
+        return json.load(f)["cfg%d" % idx]
 
 
 def kernel_flops(st, F):
-    """Algorithmic ops of the lane-group and vacuum kernel launches of one render."""
+    """Algorithmic ops of the lane-group and vacuum kernel launches of one render.  The
+    vacuum kernel only takes steps whose transfer components are frozen, so its steps are
+    priced as geodesic-only steps (the 8 geodesic components)."""
     g = st["steps_group"]
     gt = min(st["steps_transfer"], g)
     v = st["steps_attempted"] - g
-    return gt * F["flops_per_transfer_step"] + (g - gt) * F["flops_per_vacuum_step"], v * F["flops_per_vacuum_step"]
+    return (gt * F["flops_per_transfer_step"] + (g - gt) * F["flops_per_vacuum_step"],
+            v * F["flops_per_geodesic_step"])
 
 
-def traffic_per_launch():
+def traffic_per_launch(idx=3):
     """HBM bytes per lane-group launch from the committed PMC summary (None if absent)."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(TRAFFIC_SUMMARY[idx]) as f:
             m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
         return float(m.group(1)) if m else None
     except OSError:
@@ -173,21 +185,30 @@ def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
     return times, xyz, cls, stp
 
 
-def cpu_baseline(w, sample_n):
+def cpu_baseline(w, sample_n, n_poses=1):
+    """The oracle on sample_n pixels spread evenly over n_poses of the workload's cameras."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     c0 = w.cameras[0]
     npix = c0.x_pix * c0.x_pix
-    stride = max(1, npix // sample_n)
-    subset = np.arange(0, npix, stride)[:sample_n]
+    poses = list(range(0, len(w.cameras), max(1, len(w.cameras) // n_poses)))[:n_poses]
+    per = max(1, sample_n // len(poses))
+    stride = max(1, npix // per)
+    subset = np.arange(0, npix, stride)[:per]
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     nthreads = max(1, min(nthreads, 16))
-    r = O.render(w.scene, c0, w.integ, w.lam, nthreads=nthreads, subset=subset)
-    steps = int(r["steps"][subset, 0].sum())
-    return {"value": len(subset) / r["seconds"], "unit": "rays/s", "cores": nthreads, "kind": "port",
-            "steps_per_s": steps / r["seconds"], "seconds": r["seconds"],
-            "sample": "%d pixels of the 1024x1024 camera (every %dth), FP64 oracle, %d threads" %
-                      (len(subset), stride, nthreads)}
+    secs, steps, rays = 0.0, 0, 0
+    for k in poses:
+        r = O.render(w.scene, w.cameras[k], w.integ, w.lam, nthreads=nthreads, subset=subset)
+        secs += r["seconds"]
+        steps += int(r["steps"][subset, 0].sum())
+        rays += len(subset)
+    where = ("the %dx%d camera" % (c0.x_pix, c0.x_pix) if len(poses) == 1 else
+             "each of %d poses (%dx%d)" % (len(poses), c0.x_pix, c0.x_pix))
+    return {"value": rays / secs, "unit": "rays/s", "cores": nthreads, "kind": "port",
+            "steps_per_s": steps / secs, "seconds": secs,
+            "sample": "%d pixels of %s (every %dth), FP64 oracle, %d threads" %
+                      (len(subset), where, stride, nthreads)}
 
 
 def main():
@@ -205,42 +226,60 @@ def main():
     if a.adjoint_check:
         adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first)
         return
-    w = scenes.config(3, x_pix=a.x_pix, precision=abi.PREC_MIXED, schedule=a.schedule)
+    idx = WORKLOADS[a.workload]
+    x_pix = a.x_pix or (1024 if idx == 3 else 512)
+    w = scenes.config(idx, x_pix=x_pix, precision=abi.PREC_MIXED, schedule=a.schedule)
+    if idx == 5:
+        w.cameras = w.cameras[:max(1, min(a.poses, len(w.cameras)))]
     ctx = host.Context(0)
     times, xyz, cls, stp = gpu_render(ctx, w, abi.PREC_MIXED, a.steps, a.warmup, torch)
     st = ctx.last_stats()  # launch times and counters of the last timed render
     ms = float(np.mean(times))
     npix = xyz.shape[0]
     att = int(stp[:, 0].to(torch.int64).sum().item())
-    F = flops_table()
+    F = flops_table(idx)
     fg, fv = kernel_flops(st, F)
     achieved = fg / (st["group_ms"] * 1e-3) / 1e12
     nl = max(1, st["group_launches"])
+    if idx == 3:
+        metric = "rays/s and RK45 steps/s, 1024^2 Kerr; mixed-prec vs FP64 at <=1e-3 XYZ error"
+        kind = "KERR_FF"
+    else:
+        metric = "rays/s and RK45 steps/s, %d-pose %d^2 Kerr a=0.94 synchrotron torus + jet batch" % (
+            len(w.cameras), x_pix)
+        kind = "KERR_SYNCH"
     out = {
-        "metric": "rays/s and RK45 steps/s, 1024^2 Kerr; mixed-prec vs FP64 at <=1e-3 XYZ error",
+        "metric": metric,
         "value": npix / (ms * 1e-3), "unit": "rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 stages + f64 state/controller", "data": "synthetic",
-        "config": {"workload": w.name, "rays": npix, "n_lambda": len(w.lam), "rtol": w.integ.rtol,
+        "config": {"workload": w.name, "poses": len(w.cameras), "rays": npix, "n_lambda": len(w.lam),
+                   "rtol": w.integ.rtol,
                    "atol": w.integ.atol, "max_dt_scale": w.integ.max_dt_scale, "max_steps": w.integ.max_steps,
                    "precision": "mixed", "schedule": {0: "grid", 1: "persistent", 2: "wavefront"}[a.schedule]},
         "rk45_steps_per_s": att / (ms * 1e-3),
         "steps_attempted": att,
+        "steps_per_ray": {"min": int(stp[:, 0].min().item()), "median": float(stp[:, 0].double().median().item()),
+                          "max": int(stp[:, 0].max().item())},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_VALU, "traffic": traffic_per_launch(),
-                     "kernel": "rk45_kernel<KERR_FF, 8 lanes/ray, mixed> (lane-group transfer kernel)",
+                     "frac": achieved / PEAK_FP32_VALU, "traffic": traffic_per_launch(idx),
+                     "kernel": "rk45_kernel<%s, 8 lanes/ray, mixed> (lane-group transfer kernel)" % kind,
                      "launches": st["group_launches"], "avg_launch_ms": st["group_ms"] / nl,
                      "flops_per_launch": fg / nl, "steps_per_launch": st["steps_group"] / nl,
                      "flops_per_transfer_step": F["flops_per_transfer_step"],
                      "flops_per_vacuum_step": F["flops_per_vacuum_step"]},
         "vacuum_kernel": {"launches": st["vacuum_launches"], "ms": st["vacuum_ms"],
+                          "flops_per_geodesic_step": F["flops_per_geodesic_step"],
                           "steps": st["steps_attempted"] - st["steps_group"],
                           "achieved_tflops": fv / max(1e-9, st["vacuum_ms"] * 1e-3) / 1e12},
         "whole_render_tflops": (fg + fv) / (ms * 1e-3) / 1e12,
     }
     if not a.no_fp64:
         X = xyz.cpu().numpy()
-        for tag, sched in (("fp64_plain", abi.SCHED_GRID), ("fp64_wavefront", abi.SCHED_WAVEFRONT)):
+        fp64_runs = (("fp64_plain", abi.SCHED_GRID), ("fp64_wavefront", abi.SCHED_WAVEFRONT))
+        if idx == 5:  # the plain grid launch of 16.8M rays adds minutes; the wavefront FP64 is the reference
+            fp64_runs = fp64_runs[1:]
+        for tag, sched in fp64_runs:
             t64, xyz64, cls64, stp64 = gpu_render(ctx, w, abi.PREC_FP64, 1, 0, torch, schedule=sched)
             ms64 = float(np.mean(t64))
             att64 = int(stp64[:, 0].to(torch.int64).sum().item())
@@ -256,11 +295,16 @@ def main():
                         "mixed_xyz_rel_err_median": float(np.median(rel)) if m.any() else 0.0,
                         "mixed_xyz_rel_err_p99": float(np.percentile(rel, 99)) if m.any() else 0.0,
                         "class_agreement": float((cls.cpu().numpy() == cls64.cpu().numpy()).mean())}
-        out["speedup_vs_fp64_plain"] = out["fp64_plain"]["speedup_of_mixed"]
-        out["xyz_rel_err_median"] = out["fp64_plain"]["mixed_xyz_rel_err_median"]
-        out["class_agreement"] = out["fp64_plain"]["class_agreement"]
+        ref64 = out["fp64_plain"] if "fp64_plain" in out else out["fp64_wavefront"]
+        if "fp64_plain" in out:
+            out["speedup_vs_fp64_plain"] = out["fp64_plain"]["speedup_of_mixed"]
+        out["xyz_rel_err_median"] = ref64["mixed_xyz_rel_err_median"]
+        out["class_agreement"] = ref64["class_agreement"]
     if not a.no_cpu:
-        cb = cpu_baseline(scenes.config(3, x_pix=a.x_pix), a.cpu_sample)
+        wc = scenes.config(idx, x_pix=x_pix)
+        if idx == 5:
+            wc.cameras = wc.cameras[:len(w.cameras)]
+        cb = cpu_baseline(wc, a.cpu_sample if idx == 3 else min(a.cpu_sample, 16384), 1 if idx == 3 else 8)
         out["cpu_baseline"] = cb
     print(json.dumps(out))
 

@@ -52,6 +52,23 @@ enum rtw_scene_kind {
   RTW_SCENE_KERR_SYNCH = 5          /* Kerr + thermal-synchrotron torus and conical jet */
 };
 
+/* RTW_SCENE_KERR_SYNCH (no reference counterpart; the reference's nearest scene
+ * is main/Powehi.source with power-law synchrotron in a 5-degree jet):
+ *   torus  Fishbone-Moncrief, constant l = u_phi u^t taken from the prograde
+ *          circular orbit at torus_r_max, inner edge r_in on the equator,
+ *          gamma = 4/3 polytrope: n_e = density * ((h - 1) / (h_max - 1))^3;
+ *   jet    r >= r_jet and |cos theta| >= cos(jet_half_angle) (0 < angle < pi/2):
+ *          n_e = density * absorb * (r_jet / r)^2 (1 - r_jet / r) w^2 with
+ *          w = 1 - (1 - |cos theta|) / (1 - cos(jet_half_angle)), i.e. a conical
+ *          outflow whose density vanishes continuously at the base and the
+ *          wall, radial with Lorentz factor 2 in the frame of the
+ *          zero-angular-momentum observer;
+ *   matter thermal electrons at theta_e in a field of bfield Gauss (pitch angle
+ *          60 degrees), emissivity of Leung, Gammie & Noble (2011, eq. 31),
+ *          absorption by Kirchhoff's law; lengths in units of M = 9.62e14 cm
+ *          (the map scale of the reference's M87* model).
+ * temperature, r_out, th_lo/th_hi, h_over_r and z_cut are unused. */
+
 /* RTW_SCENE_REF_F32: evaluate the Kerr family as the reference's Float64 mode
  * does, i.e. Christoffel symbols in Float32 (main/Kerr.source/source_geometry.jl:196-200),
  * spin/horizon/ISCO constants in Float32 (:7-37) and Float32(max_dt_scale)
@@ -73,9 +90,11 @@ typedef struct rtw_scene {
   double mass;           /* M (geometric units) */
   double spin;           /* a */
   double temperature;    /* K */
-  double absorb;         /* constant a_nu (blackbody scenes) or opacity scale kappa (free-free) */
-  double density;        /* density scale rho_s (free-free) / electron density scale (synchrotron) */
-  double r_in, r_out;    /* radial extent of the emitting matter */
+  double absorb;         /* constant a_nu (blackbody scenes), opacity scale kappa (free-free),
+                            jet density at r_jet over n_0 (synchrotron) */
+  double density;        /* density scale rho_s (free-free), n_0 in cm^-3 at the torus
+                            pressure maximum (synchrotron) */
+  double r_in, r_out;    /* radial extent of the emitting matter (synchrotron: torus inner edge) */
   double th_lo, th_hi;   /* theta wedge of the thin disks */
   double h_over_r;       /* thick disk scale height H / r */
   double z_cut;          /* thick disk truncation |z| <= z_cut * H */

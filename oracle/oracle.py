@@ -55,6 +55,9 @@ def lib():
         L.rtwo_count_ops.argtypes = [P, P, P, P, C.c_int, P, C.c_int64, P]
         L.rtwo_derivative.argtypes = [P, P, P, C.c_int, P, P]
         L.rtwo_ray_xyz.argtypes = [P, P, C.c_int, P]
+        L.rtwo_synch_derived.argtypes = [P, P]
+        L.rtwo_synch_point.argtypes = [P, P, P]
+        L.rtwo_radiative.argtypes = [P, P, D, P]
         rc = L.rtwo_load_gaunt(GAUNT_PATH.encode())
         if rc != 0:
             raise RuntimeError("oracle: cannot load Gaunt table (%d)" % rc)
@@ -90,6 +93,26 @@ def scene_derived(scene):
     out = np.zeros(3)
     lib().rtwo_scene_derived(_r(scene), _p(out))
     return {"r_s": out[0], "r_ev": out[1], "spin": out[2]}
+
+
+def synch_derived(scene):
+    out = np.zeros(6)
+    lib().rtwo_synch_derived(_r(scene), _p(out))
+    return dict(zip(("ell", "F_in", "hm1max", "T_e", "nu_s", "Q"), out))
+
+
+def synch_point(scene, x):
+    """(region 0/1/2 = vacuum/torus/jet, n_e / n_0, ln h) at x."""
+    out = np.zeros(3)
+    lib().rtwo_synch_point(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(out))
+    return int(out[0]), out[1], out[2]
+
+
+def radiative(scene, x, nu):
+    """(a_nu, j_nu) at x for fluid-frame frequency nu (nm^-1)."""
+    out = np.zeros(2)
+    lib().rtwo_radiative(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), float(nu), _p(out))
+    return out[0], out[1]
 
 
 def metric(scene, x):

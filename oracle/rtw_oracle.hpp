@@ -161,6 +161,10 @@ struct Scene {
   double torus_r_max = 12.0;         // synchrotron: torus pressure maximum
   // derived by scene_init
   double r_s = 2.0, r_ev = 0.0;
+  // synchrotron torus + jet (scene_init): torus angular momentum u_phi u^t, its
+  // potential at the inner edge, h - 1 at the pressure maximum; electron
+  // temperature; synchrotron frequency nu_s; absorption prefactor Q (per M)
+  double syn_ell = 0, syn_Fin = 0, syn_hm1max = 1, syn_Te = 0, syn_nus = 0, syn_Q = 0;
 };
 
 // calc_orb_limits prograde ISCO exactly as Kerr.source/source_geometry.jl:20-26 (Q5:
@@ -188,6 +192,7 @@ inline double kerr_circ_beta(double r, double a, double M) {
   return std::sqrt(M) * (r * r - 2 * a * std::sqrt(M * r) + a * a) / (std::sqrt(D) * (std::pow(r, 1.5) + a * std::sqrt(M)));
 }
 
+inline void synch_init(Scene& s);
 inline void scene_init(Scene& s) {
   s.r_s = 2 * s.mass;
   if (s.kind == SCHWARZSCHILD_DISK) {
@@ -204,6 +209,7 @@ inline void scene_init(Scene& s) {
   } else {
     s.r_ev = 0.0;
   }
+  if (s.kind == KERR_SYNCH) synch_init(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,6 +465,90 @@ inline double gaunt_van_hoof_f64(double gam2, double u) {
   return 1.2;
 }
 
+// ---------------------------------------------------------------------------
+// Thermal-synchrotron torus + jet (new scene, Kerr geometry, M = mass):
+//   torus: Fishbone & Moncrief (1976) constant l = u_phi u^t, l from the prograde
+//     circular orbit at the pressure maximum torus_r_max (Bardeen, Press &
+//     Teukolsky 1972), inner edge r_in on the equator; ln h = F(r, th) - F(r_in,
+//     pi/2) with F = 1/2 ln[(1 + q) A / (Sigma Delta)] - q/2 - 2 a M r l / A,
+//     q = sqrt(1 + 4 l^2 Sigma^2 Delta / (A^2 sin^2 th)); polytrope gamma = 4/3, so
+//     n_e / n_0 = ((h - 1) / (h_max - 1))^3; flow u^t = gamma_Z / alpha,
+//     u^phi = gamma_Z omega / alpha + U / sqrt(g_phiphi), U^2 = (-1 + sqrt(1 +
+//     4 s^2)) / 2, s^2 = l^2 Sigma^2 Delta / (A^2 sin^2 th);
+//   jet: r >= r_jet and |cos th| >= cos(jet_half_angle), n_e / n_0 = absorb *
+//     (r_jet / r)^2 (1 - r_jet / r) w^2, w = 1 - (1 - |cos th|) / (1 - cos(half-angle))
+//     (conical outflow, density continuous at the base and the wall), radial
+//     outflow with Lorentz factor 2 in the ZAMO frame;
+//   emission: thermal synchrotron of Leung, Gammie & Noble (2011, eq. 31),
+//     B = bfield (G), Theta_e = theta_e, pitch angle 60 degrees; absorption by
+//     Kirchhoff's law at T_e = Theta_e m_e c^2 / k; lengths in units of M with M
+//     the mass of M87* (the reference's map_scale, Powehi.source/source_model.jl:6).
+constexpr double CGS_E = 4.80320471e-10, CGS_ME = 9.1093837015e-28, CGS_C = 2.99792458e10;
+constexpr double CGS_H = 6.62607015e-27, CGS_K = 1.380649e-16;
+constexpr double SYN_CM_PER_M = 9.62e14;
+constexpr double SYN_SIN_PITCH = 0.86602540378443865;
+constexpr double SYN_JET_GAMMA = 2.0;
+constexpr double SYN_2_11_12 = 1.8877486253633870;  // 2^(11/12)
+
+inline void synch_init(Scene& s) {
+  const double M = s.mass, a = s.spin, r = s.torus_r_max;
+  const double den = std::sqrt(std::pow(r, 1.5) - 3 * M * std::sqrt(r) + 2 * a * std::sqrt(M));
+  const double ut = (std::pow(r, 1.5) + a * std::sqrt(M)) / (std::pow(r, 0.75) * den);
+  const double uphi = std::sqrt(M) * (r * r - 2 * a * std::sqrt(M * r) + a * a) / (std::pow(r, 0.75) * den);
+  s.syn_ell = uphi * ut;
+  s.syn_Fin = 0;
+  auto F = [&](double rr, double th) {
+    const double s2 = std::sin(th) * std::sin(th), c2 = 1 - s2;
+    const double Sg = rr * rr + a * a * c2, Dl = rr * rr - 2 * M * rr + a * a;
+    const double A = (rr * rr + a * a) * (rr * rr + a * a) - Dl * a * a * s2;
+    const double q = std::sqrt(1 + 4 * s.syn_ell * s.syn_ell * Sg * Sg * Dl / (A * A * s2));
+    return 0.5 * std::log((1 + q) * A / (Sg * Dl)) - 0.5 * q - 2 * a * M * rr * s.syn_ell / A;
+  };
+  s.syn_Fin = F(s.r_in, PI / 2);
+  s.syn_hm1max = std::expm1(F(s.torus_r_max, PI / 2) - s.syn_Fin);
+  s.syn_Te = s.theta_e * CGS_ME * CGS_C * CGS_C / CGS_K;
+  const double nuc = CGS_E * s.bfield / (2 * PI * CGS_ME * CGS_C);
+  s.syn_nus = (2.0 / 9.0) * nuc * s.theta_e * s.theta_e * SYN_SIN_PITCH;
+  const double K2 = std::cyl_bessel_k(2.0, 1.0 / s.theta_e);
+  s.syn_Q = s.density * SYN_CM_PER_M * std::sqrt(2.0) * PI * CGS_E * CGS_E * CGS_C /
+            (6 * CGS_H * K2 * s.syn_nus * s.syn_nus);
+}
+
+// ln h of the torus at x: F(r, th) - F(r_in, pi/2); NaN inside the horizon and
+// -inf on the axis.
+template <class R> R synch_lnh(const Scene& s, const R* x) {
+  const double M = s.mass, a = s.spin;
+  const R r = x[1], th = x[2];
+  const R sn = sin(th), cs = cos(th);
+  const R s2 = sn * sn, c2 = cs * cs;
+  const R Sg = r * r + R(a * a) * c2, Dl = r * r - R(2 * M) * r + R(a * a);
+  const R A = sq(r * r + R(a * a)) - Dl * R(a * a) * s2;
+  const R q = sqrt(R(1) + R(4 * s.syn_ell * s.syn_ell) * Sg * Sg * Dl / (A * A * s2));
+  return R(0.5) * log((R(1) + q) * A / (Sg * Dl)) - R(0.5) * q - R(2 * a * M) * r * R(s.syn_ell) / A - R(s.syn_Fin);
+}
+
+// Region (0 vacuum, 1 torus, 2 jet) and n_e / n_0 at x.
+template <class R> int synch_region(const Scene& s, const R* x, R& dens) {
+  const R r = x[1];
+  dens = R(0);
+  if (r >= R(s.r_in) && sin(x[2]) != R(0) && sq(r) - R(2 * s.mass) * r + R(s.spin * s.spin) > R(0)) {
+    const R lnh = synch_lnh(s, x);
+    if (lnh > R(0)) {
+      const R f = (exp(lnh) - R(1)) / R(s.syn_hm1max);
+      dens = f * f * f;
+      return 1;
+    }
+  }
+  const double cj = std::cos(s.jet_half_angle);
+  if (r >= R(s.r_jet) && fabs(cos(x[2])) >= R(cj)) {
+    const R q = R(s.r_jet) / r;
+    const R w = R(1) - (R(1) - fabs(cos(x[2]))) / R(1 - cj);
+    dens = R(s.absorb) * q * q * (R(1) - q) * w * w;
+    return 2;
+  }
+  return 0;
+}
+
 // Matter region test (is_fire).  Minkowski source_model.jl:253-260 (r <= 1),
 // Schwarzschild :273-280 and Kerr :73-80 (wedges); the thick disk (new) is a
 // radial band with |cos theta| <= z_cut * h_over_r.
@@ -470,8 +560,46 @@ template <class R> bool is_fire(const Scene& s, const R* x) {
       return R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && R(s.th_lo) <= x[2] && x[2] <= R(s.th_hi);
     case KERR_THICK_FF:
       return R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && fabs(cos(x[2])) <= R(s.z_cut * s.h_over_r);
+    case KERR_SYNCH: {
+      R d;
+      return synch_region(s, x, d) != 0;
+    }
     default: return false;
   }
+}
+
+// FIDO (zero-angular-momentum observer) of the Kerr family, Kerr.source/source_model.jl:122-134.
+template <class R> void kerr_fido(const Scene& s, const R* x, R u[4]) {
+  const R M(s.mass), a(s.spin);
+  R A = sqrt(sq(x[1]) + sq(a * cos(x[2])));
+  R B = sq(x[1]) + sq(a) - R(2) * M * x[1];
+  R C = sq(sq(x[1]) + sq(a)) - B * sq(a * sin(x[2]));
+  R ut = sqrt(C) / (A * sqrt(B));
+  R Om = R(2) * a * M * x[1] / C;
+  u[0] = ut; u[1] = R(0); u[2] = R(0); u[3] = ut * Om;
+}
+
+// Synchrotron scene flow in region reg (1 torus, 2 jet); false in vacuum (FIDO).
+template <class R> bool synch_velocity(const Scene& s, const R* x, int reg, R u[4]) {
+  if (reg == 0) return false;
+  const R M(s.mass), a(s.spin);
+  const R r = x[1], sn = sin(x[2]), cs = cos(x[2]);
+  const R s2 = sn * sn, c2 = cs * cs;
+  const R Sg = sq(r) + sq(a) * c2, Dl = sq(r) - R(2) * M * r + sq(a);
+  const R A = sq(sq(r) + sq(a)) - Dl * sq(a) * s2;
+  const R alpha = sqrt(Sg * Dl / A), omega = R(2) * a * M * r / A;
+  if (reg == 1) {  // Fishbone-Moncrief torus flow
+    const R ss = R(s.syn_ell * s.syn_ell) * Sg * Sg * Dl / (A * A * s2);
+    const R U = sqrt((R(-1) + sqrt(R(1) + R(4) * ss)) / R(2));
+    const R gz = sqrt(R(1) + U * U);
+    u[0] = gz / alpha; u[1] = R(0); u[2] = R(0);
+    u[3] = gz * omega / alpha + U / sqrt(A * s2 / Sg);
+  } else {  // jet: radial outflow, Lorentz factor SYN_JET_GAMMA in the ZAMO frame
+    const R G(SYN_JET_GAMMA);
+    const R v = sqrt(R(1) - R(1) / (G * G));
+    u[0] = G / alpha; u[1] = G * v * sqrt(Dl / Sg); u[2] = R(0); u[3] = G * omega / alpha;
+  }
+  return true;
 }
 
 // get_source_velocity!: Minkowski source_model.jl:288-294, Schwarzschild :312-333,
@@ -494,6 +622,10 @@ template <class R> void source_velocity(const Scene& s, const R* x, const R g[4]
     return;
   }
   R a(s.spin);
+  if (s.kind == KERR_SYNCH) {
+    R d;
+    if (synch_velocity(s, x, synch_region(s, x, d), u)) return;
+  }
   bool keplerian = (s.kind == KERR_SYNCH) ? false : is_fire(s, x);
   if (keplerian) {
     R Om = sqrt(M) / (pow(x[1], R(1.5)) + a * sqrt(M));
@@ -501,12 +633,7 @@ template <class R> void source_velocity(const Scene& s, const R* x, const R g[4]
     R ut = (pow(x[1], R(1.5)) + a * sqrt(M)) / (pow(x[1], R(0.75)) * sqrt(D));
     u[0] = ut; u[1] = R(0); u[2] = R(0); u[3] = ut * Om;
   } else {
-    R A = sqrt(sq(x[1]) + sq(a * cos(x[2])));
-    R B = sq(x[1]) + sq(a) - R(2) * M * x[1];
-    R C = sq(sq(x[1]) + sq(a)) - B * sq(a * sin(x[2]));
-    R ut = sqrt(C) / (A * sqrt(B));
-    R Om = R(2) * a * M * x[1] / C;
-    u[0] = ut; u[1] = R(0); u[2] = R(0); u[3] = ut * Om;
+    kerr_fido(s, x, u);
   }
 }
 
@@ -535,6 +662,12 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
       fp[1] = R(0);
     }
   }
+  if (s.kind == KERR_SYNCH) {  // electron temperature, n_e / n_0, region
+    R d;
+    fp[2] = R((double)synch_region(s, x, d));
+    fp[0] = R(s.syn_Te);
+    fp[1] = d;
+  }
 }
 
 // Thick-disk free-free constants: reference temperature and frequency of the
@@ -554,6 +687,25 @@ template <class R> void radiative_coefficients(const Scene& s, const R* x, const
       if (std::is_same<R, Cnt>::value) { op_count().arith += 14; op_count().trans += 2; }
       a_nu = R(s.absorb) * fp[1] * sqrt(R(FF_T0) / T) * cube(R(FF_NU0) / nu) * (R(1) - exp(-u)) * R((double)gff);
       j_nu = a_nu * planck(T, nu);
+    } else {
+      a_nu = R(0); j_nu = R(0);
+    }
+    return;
+  }
+  if (s.kind == KERR_SYNCH) {
+    if (val(fp[2]) != 0.0) {  // region from fluid_params
+      // X = nu / nu_s (nu in the fluid frame), G(X) = (X^1/2 + 2^(11/12) X^1/6)^2
+      // exp(-X^1/3).  In the units of the blackbody scenes (j / nu^3 in units of
+      // EMISSION_SCALE) Kirchhoff's law gives j / nu^3 = EMISSION_SCALE Q n G / X^3
+      // and alpha = Q n G expm1(h nu / k T_e) / X^3.
+      const R X = nu * R(CGS_C * 1e7 / s.syn_nus);
+      const R x3 = pow(X, R(1.0 / 3.0));
+      const R g = sqrt(X) + R(SYN_2_11_12) * sqrt(x3);
+      const R q = R(s.syn_Q) * fp[1] * (g * g) * exp(-x3) / (X * X * X);
+      const R em = R(std::expm1(val(R(F_FACTOR) * nu / fp[0])));
+      if (std::is_same<R, Cnt>::value) { op_count().trans += 1; op_count().arith += 2; }
+      a_nu = q * em;
+      j_nu = q * R(EMISSION_SCALE) * cube(nu);
     } else {
       a_nu = R(0); j_nu = R(0);
     }
@@ -619,9 +771,17 @@ template <class R> struct RaySys {
       slope[i] = a;
     }
     lower_metric(*sc, Ray, g);
-    source_velocity(*sc, Ray, g, u);
+    if (sc->kind == KERR_SYNCH) {  // region and density once per evaluation
+      R d;
+      const int reg = synch_region(*sc, Ray, d);
+      if (!synch_velocity(*sc, Ray, reg, u)) kerr_fido(*sc, Ray, u);
+      fp[0] = R(sc->syn_Te);
+      fp[1] = d;
+      fp[2] = R((double)reg);
+    }
+    else source_velocity(*sc, Ray, g, u);
     R fs = freq_shift(Ray, u, g);
-    fluid_params(*sc, Ray, u, g, fs, fp);
+    if (sc->kind != KERR_SYNCH) fluid_params(*sc, Ray, u, g, fs, fp);
     for (int c = 0; c < ncol; ++c) {
       int i = 8 + 2 * c;
       R nu = R(freq[c]) * fs;

@@ -115,6 +115,33 @@ void rtwo_keepinbounds(const rtw_scene* s, double* ray8) {
   keepinbounds<double>(sc, ray8);
 }
 
+// Synchrotron scene: derived constants (l, F_in, h_max - 1, T_e, nu_s, Q) and,
+// at x, (region, n_e / n_0, ln h).
+void rtwo_synch_derived(const rtw_scene* s, double* out6) {
+  Scene sc = to_scene(s);
+  out6[0] = sc.syn_ell; out6[1] = sc.syn_Fin; out6[2] = sc.syn_hm1max;
+  out6[3] = sc.syn_Te; out6[4] = sc.syn_nus; out6[5] = sc.syn_Q;
+}
+void rtwo_synch_point(const rtw_scene* s, const double* x, double* out3) {
+  Scene sc = to_scene(s);
+  double d = 0;
+  out3[0] = synch_region(sc, x, d);
+  out3[1] = d;
+  out3[2] = synch_lnh(sc, x);
+}
+// Absorption and emission coefficients (a_nu, j_nu) at x for the fluid-frame
+// frequency nu (nm^-1), with the scene's fluid parameters at x.
+void rtwo_radiative(const rtw_scene* s, const double* x, double nu, double* out2) {
+  Scene sc = to_scene(s);
+  double g[4][4], u[4], fp[8] = {0};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) g[i][j] = (i == j);
+  lower_metric<double>(sc, x, g);
+  source_velocity<double>(sc, x, g, u);
+  fluid_params<double>(sc, x, u, g, 1.0, fp);
+  radiative_coefficients<double>(sc, x, fp, nu, out2[0], out2[1]);
+}
+
 void rtwo_source_velocity(const rtw_scene* s, const double* x, double* u4) {
   Scene sc = to_scene(s);
   double g[4][4];
@@ -323,7 +350,15 @@ int rtwo_count_ops(const rtw_scene* s, const rtw_camera* c, const rtw_integrator
       }
       sc_ += 1;
       steps++;
-      if (sys.terminate(ray.data(), op.max_steps, sc_)) break;
+      if (nlam == 0) {
+        // geodesic-only count (no colours): stop on the geometric conditions of
+        // calc_terminate alone, since "every colour opaque" holds vacuously
+        const double r = val(ray[1]);
+        const bool horizon = sc.kind != MINKOWSKI_SPHERE && r <= (1 + 10 * op.rtol) * sc.r_ev;
+        if (sc_ >= op.max_steps || r > sc.r_escape || horizon || val(ray[3]) > 8 * PI || !std::isfinite(r)) break;
+      } else if (sys.terminate(ray.data(), op.max_steps, sc_)) {
+        break;
+      }
     }
   }
   out6[0] = (double)(op_count().arith - ar0);

@@ -246,6 +246,55 @@ static void cam_derivs(const DevScene& S, const rtw_camera& c, double dMa[16], d
   }
 }
 
+// Thermal-synchrotron torus + jet: constants of the Fishbone-Moncrief torus
+// (l = u_phi u^t of the prograde circular orbit at the pressure maximum, Bardeen,
+// Press & Teukolsky 1972) and of the Leung, Gammie & Noble (2011) emissivity with
+// Kirchhoff absorption, lengths in units of M = M87*'s gravitational radius.
+constexpr double CGS_E = 4.80320471e-10, CGS_ME = 9.1093837015e-28, CGS_C = 2.99792458e10;
+constexpr double CGS_H = 6.62607015e-27, CGS_K = 1.380649e-16;
+constexpr double SYN_CM_PER_M = 9.62e14;  // map scale of the reference's M87* model
+constexpr double SYN_SIN_PITCH = 0.86602540378443865;  // pitch angle 60 degrees
+struct SynchDerived {
+  double ell, Fin, hm1max, Te, nus, Q;
+};
+static double fm_potential(double M, double a, double ell, double r, double th) {
+  const double s2 = std::sin(th) * std::sin(th), c2 = 1 - s2;
+  const double Sg = r * r + a * a * c2, Dl = r * r - 2 * M * r + a * a;
+  const double A = (r * r + a * a) * (r * r + a * a) - Dl * a * a * s2;
+  const double q = std::sqrt(1 + 4 * ell * ell * Sg * Sg * Dl / (A * A * s2));
+  return 0.5 * std::log((1 + q) * A / (Sg * Dl)) - 0.5 * q - 2 * a * M * r * ell / A;
+}
+static bool synch_derive(const rtw_scene& s, double a, SynchDerived& o, const char** why) {
+  const double M = s.mass, r = s.torus_r_max;
+  const double den2 = std::pow(r, 1.5) - 3 * M * std::sqrt(r) + 2 * a * std::sqrt(M);
+  if (!(s.r_in > 0 && r > s.r_in && den2 > 0)) {
+    *why = "torus needs 0 < r_in < torus_r_max outside the photon orbit";
+    return false;
+  }
+  if (!(s.bfield > 0 && s.theta_e > 0 && s.density >= 0 && s.absorb >= 0 && s.jet_half_angle > 0 &&
+        s.jet_half_angle < PI / 2 && s.r_jet > 0)) {
+    *why = "synchrotron scene needs bfield > 0, theta_e > 0, density >= 0, absorb >= 0, "
+           "0 < jet_half_angle < pi/2, r_jet > 0";
+    return false;
+  }
+  const double den = std::sqrt(den2);
+  const double ut = (std::pow(r, 1.5) + a * std::sqrt(M)) / (std::pow(r, 0.75) * den);
+  const double uphi = std::sqrt(M) * (r * r - 2 * a * std::sqrt(M * r) + a * a) / (std::pow(r, 0.75) * den);
+  o.ell = uphi * ut;
+  o.Fin = fm_potential(M, a, o.ell, s.r_in, PI / 2);
+  o.hm1max = std::expm1(fm_potential(M, a, o.ell, r, PI / 2) - o.Fin);
+  if (!(o.hm1max > 0)) {
+    *why = "torus parameters give no torus (h <= 1 at the pressure maximum)";
+    return false;
+  }
+  o.Te = s.theta_e * CGS_ME * CGS_C * CGS_C / CGS_K;
+  const double nuc = CGS_E * s.bfield / (2 * PI * CGS_ME * CGS_C);
+  o.nus = (2.0 / 9.0) * nuc * s.theta_e * s.theta_e * SYN_SIN_PITCH;
+  const double K2 = std::cyl_bessel_k(2.0, 1.0 / s.theta_e);
+  o.Q = s.density * SYN_CM_PER_M * std::sqrt(2.0) * PI * CGS_E * CGS_E * CGS_C / (6 * CGS_H * K2 * o.nus * o.nus);
+  return true;
+}
+
 // Scene constants (main/Kerr.source/source_geometry.jl:7-17 and the scene model files).
 static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o, DevScene& d) {
   std::memset(&d, 0, sizeof(d));
@@ -263,7 +312,7 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.r_escape = s.r_escape;
   d.r_photon15 = 1.5 * (3 * s.mass);
   d.ref_f32 = (s.kind >= RTW_SCENE_KERR_DISK && (s.flags & RTW_SCENE_REF_F32)) ? 1 : 0;
-  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_KERR_THICK_FF)
+  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_KERR_SYNCH)
     return fail(ctx, RTW_E_UNSUPPORTED, "scene kind not supported");
   if (s.kind == RTW_SCENE_SCHWARZSCHILD_DISK) {
     d.r_ev = d.rs;
@@ -286,6 +335,22 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.eps_pole = mds < tenth_f ? mds : tenth_f;
   d.eps_photon = o.max_dt_scale / 10;
   d.bb_a = s.absorb;
+  if (s.kind == RTW_SCENE_KERR_SYNCH) {
+    if (d.ref_f32) return fail(ctx, RTW_E_ARG, "the synchrotron scene has no Float32 reference mode");
+    SynchDerived y;
+    const char* why = nullptr;
+    if (!synch_derive(s, d.a, y, &why)) return fail(ctx, RTW_E_ARG, why);
+    d.sy_ell = y.ell;
+    d.sy_Fin = y.Fin;
+    d.sy_ihm = 1.0 / y.hm1max;
+    d.sy_Q = y.Q;
+    d.sy_l2Q = std::log2(y.Q);
+    d.sy_rjet = s.r_jet;
+    d.sy_cjet = std::cos(s.jet_half_angle);
+    d.sy_iwj = 1.0 / (1.0 - d.sy_cjet);
+    d.T = y.Te;  // colour constants use the electron temperature
+    d.sy_nus = y.nus;
+  }
   if (s.kind == RTW_SCENE_KERR_THICK_FF) {
     if (!ctx->gaunt_loaded) return fail(ctx, RTW_E_ARG, "free-free scene needs rtw_load_gaunt first");
     d.ff_C = s.absorb * std::sqrt(FF_T0 / s.temperature);
@@ -338,6 +403,12 @@ static std::vector<DevColor> make_colors(const DevScene& S, const double* lam, i
     c.lc = std::log10(c.uf);
     double r = FF_NU0 / c.f;
     c.if3 = r * r * r;
+    if (S.kind == KERR_SYNCH) {
+      // x3_c = (nu / nu_s)^(1/3) at unit shift (f in nm^-1 -> 1e7 cm^-1) and
+      // K_c = log2 Q - x3_c log2(e) - 8 log2 x3_c
+      c.if3 = std::cbrt(c.f * CGS_C * 1e7 / S.sy_nus);
+      c.lc = S.sy_l2Q - c.if3 / std::log(2.0) - 8 * std::log2(c.if3);
+    }
     c.cx = c.f * c.f * c.f * C_LIGHT / (lam[i] * lam[i]);
     double w = step;
     if (i == 0) w /= 2;

@@ -49,6 +49,11 @@ struct DevScene {
   int g_ok_j;          // Gaunt column indices in range
   double gw0d, gw1d;   // the same weights in FP64 (SCENE_GAUNT_FP64)
   int ref_f32;         // Float32 Christoffel symbols
+  // synchrotron torus + jet: torus l = u_phi u^t, potential at the inner edge,
+  // 1 / (h_max - 1); absorption prefactor Q (per M at n_e = n_0) and log2 Q;
+  // jet base radius, cos(half-angle) and 1 / (1 - cos(half-angle))
+  double sy_ell, sy_Fin, sy_ihm, sy_Q, sy_l2Q, sy_rjet, sy_cjet, sy_iwj;
+  double sy_nus;  // synchrotron frequency nu_s (Hz), host side only
 };
 
 struct DevOpts {
@@ -60,8 +65,8 @@ struct DevOpts {
 struct DevColor {
   double f;     // 1 / lambda (nm^-1)
   double uf;    // F_FACTOR f / T  (u = uf * shift)
-  double lc;    // log10(uf)
-  double if3;   // (nu0 / f)^3
+  double lc;    // free-free: log10(uf); synchrotron: K = log2 Q - x3 log2(e) - 8 log2 x3
+  double if3;   // free-free: (nu0 / f)^3; synchrotron: x3 = (nu / nu_s)^(1/3) at unit shift
   double cx;    // f^3 c / lambda^2: I_lambda = (I/nu^3) * cx  (color_theory.jl:73)
   double wx, wy, wz;  // CIE weights (color_theory.jl:79-91)
 };
@@ -116,6 +121,22 @@ __device__ __forceinline__ double exp_(double x) { return exp(x); }
 __device__ __forceinline__ float exp_(float x) { return __expf(x); }
 __device__ __forceinline__ double log10_(double x) { return log10(x); }
 __device__ __forceinline__ float log10_(float x) { return __log10f(x); }
+__device__ __forceinline__ double log_(double x) { return log(x); }
+__device__ __forceinline__ float log_(float x) { return __logf(x); }
+// expm1: libm in double; in float a 4-term series below 0.03 (relative error
+// < 2e-8) and exp(x) - 1 above, selected without a branch.
+__device__ __forceinline__ double expm1_(double x) { return expm1(x); }
+__device__ __forceinline__ float expm1_(float x) {
+  const float s = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f))));
+  const float e = __expf(x) - 1.0f;
+  return fabsf(x) < 0.03f ? s : e;
+}
+
+// Thermal synchrotron (Leung, Gammie & Noble 2011, eq. 31): with x3 = X^(1/3),
+// X = nu / nu_s, the emissivity's X-dependence G(X) / X^3 =
+// (X^1/2 + 2^(11/12) X^1/6)^2 exp(-X^1/3) / X^3 = (x3 + 2^(11/12))^2 exp(-x3) / x3^8.
+constexpr double SYN_2_11_12 = 1.8877486253633870;
+constexpr double SYN_JET_GAMMA = 2.0;  // jet Lorentz factor in the ZAMO frame
 
 // Reciprocal and square root: IEEE in double and in the reference's Float32 paths
 // (FAST = false); the hardware v_rcp_f32 / v_sqrt_f32 (1 ulp) for the FP32 stages
@@ -154,7 +175,7 @@ template <bool FAST> __device__ __forceinline__ double ctrl_factor(double err) {
 template <class T> struct StageOut {
   T d[8];      // dx/dlambda
   T sh;        // frequency shift -u_j g_ij k^i (gen_geometry.jl:373-390)
-  T w;         // thick disk: rho^2 at the point
+  T w;         // thick disk: rho^2 at the point; synchrotron: n_e / n_0
   bool matter; // is_fire
 };
 
@@ -305,7 +326,59 @@ __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T 
     T gtp = -rs * a * r * s2 * iSg;
     T gpp = A * iSg * s2;
     bool m;
-    if constexpr (KIND == KERR_FF) {
+    if constexpr (KIND == KERR_SYNCH) {
+      // Fishbone-Moncrief torus: ln h = F(r, th) - F(r_in, pi/2),
+      // F = 1/2 ln[(1 + q) A / (Sigma Delta)] - q/2 - 2 a M r l / A,
+      // q = sqrt(1 + 4 s^2), s^2 = l^2 Sigma^2 Delta / (A^2 sin^2 th); then the jet
+      // cone r >= r_jet, |cos th| >= cos(half-angle).  NaN (inside the horizon, on
+      // the axis) compares false: no torus there.
+      const T ell = T(S.sy_ell);
+      const T iA = rcp_<F>(A), iD = rcp_<F>(D);
+      const T ss4 = T(4) * ell * ell * Sg * Sg * D * iA * iA * rcp_<F>(s2);
+      const T q = sqrt_<F>(T(1) + ss4);
+      const T om = T(2) * a * M * r * iA;
+      // ln h is a difference of O(1) terms that is at most ~0.02 inside the torus and
+      // the density goes as (ln h)^3: in FP32 stages it is formed in double from the
+      // FP32 coordinates, so that the density carries no FP32 rounding noise
+      double lnh;
+      if constexpr (sizeof(T) == 4) {
+        const double rd = r, ad = S.a, cd = c, s2d = s2;
+        const double Sgd = rd * rd + ad * ad * cd * cd, Dd = rd * rd - S.rs * rd + ad * ad;
+        const double Ad = (rd * rd + ad * ad) * Sgd + S.rs * rd * ad * ad * s2d;
+        const double iAd = 1.0 / Ad;
+        const double qd = sqrt(1.0 + 4.0 * S.sy_ell * S.sy_ell * Sgd * Sgd * Dd * iAd * iAd / s2d);
+        lnh = 0.5 * log((1.0 + qd) * Ad / (Sgd * Dd)) - 0.5 * qd - 2.0 * ad * S.M * rd * iAd * S.sy_ell - S.sy_Fin;
+      } else {
+        lnh = (double)(T(0.5) * log_((T(1) + q) * A * iSg * iD) - T(0.5) * q - om * ell - T(S.sy_Fin));
+      }
+      const bool tor = r >= T(S.r_in) && lnh > 0.0;
+      const bool jet = !tor && r >= T(S.sy_rjet) && fabs(c) >= T(S.sy_cjet);
+      m = tor || jet;
+      o.matter = m;
+      const T ia = sqrt_<F>(A * iSg * iD);  // 1 / lapse: u^t of the FIDO
+      T ut = ia, up = ia * om, ur = T(0);
+      if (tor) {  // n / n_0 = ((h - 1) / (h_max - 1))^3; flow of constant l
+        const double f = expm1(lnh) * S.sy_ihm;
+        o.w = T(f * f * f);
+        const T gz = sqrt_<F>(T(0.5) * (q + T(1)));
+        const T U = sqrt_<F>(T(0.5) * (q - T(1)));
+        ut = gz * ia;
+        up = gz * om * ia + U * sqrt_<F>(Sg * rcp_<F>(A * s2));
+      } else if (jet) {  // n / n_0 = ratio (r_jet / r)^2 (1 - r_jet / r) w^2, radial outflow
+        // 1 - r_jet / r as (r - r_jet) / r: >= 0 whenever r >= r_jet passed, also with
+        // the approximate FP32 reciprocal (whose r_jet / r can exceed 1 by an ulp)
+        const T ir = rcp_<F>(r);
+        const T x = T(S.sy_rjet) * ir;
+        const T wj = T(1) - (T(1) - fabs(c)) * T(S.sy_iwj);
+        o.w = T(S.absorb) * x * x * ((r - T(S.sy_rjet)) * ir) * wj * wj;
+        constexpr double G = SYN_JET_GAMMA;
+        ut = T(G) * ia;
+        up = ut * om;
+        ur = T(G * __builtin_sqrt(1.0 - 1.0 / (G * G))) * sqrt_<F>(D * iSg);
+      }
+      o.sh = -(ut * (gtt * kt + gtp * kph) + up * (gtp * kt + gpp * kph) + ur * (Sg * iD) * kr);
+      return;
+    } else if constexpr (KIND == KERR_FF) {
       m = T(S.r_in) <= r && r <= T(S.r_out) && fabs(c) <= T(S.cos_cut);
       if (m) {
         T rho = T(S.density) * exp_(-(c * c) * T(S.inv_2h2));
@@ -524,13 +597,13 @@ __device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, i
 // Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
 struct StageSc {
   double sh;   // shift
-  double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4
-  double lg;   // free-free: log10(shift)
+  double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4; synchrotron: -Q n / shift
+  double lg;   // free-free: log10(shift); synchrotron: shift^(1/3)
   double vac;  // 1 if no matter at this stage
 };
 // The same in FP32, for the mixed-precision stages.
 struct StageF {
-  float sh, P, lg, vac;
+  float sh, P, lg, vac;  // synchrotron: P = log2(n / n_0) - 11/3 log2(shift), lg = shift^(1/3) - 1
 };
 
 }  // namespace rtw

@@ -151,6 +151,11 @@ __device__ __forceinline__ StageSc make_stage(const DevScene& S, const StageOut<
       T q = T(-1) / o.sh;
       ss.P = (double)(q * T(S.ff_C) * o.w / (o.sh * o.sh * o.sh));
       ss.lg = (double)log10_(o.sh);
+    } else if constexpr (KIND == KERR_SYNCH) {
+      // -Q (n / n_0) / shift, in double whatever T is (Q n_0 is beyond FP32 range),
+      // and shift^(1/3)
+      ss.P = -(S.sy_Q * (double)o.w) / sh;
+      ss.lg = cbrt(sh);
     } else {
       ss.P = (-1.0 / sh) * S.bb_a;
     }
@@ -172,6 +177,13 @@ __device__ __forceinline__ StageF make_stage_f(const DevScene& S, const StageOut
     if constexpr (KIND == KERR_FF) {
       ss.P = q * float(S.ff_C) * o.w * rcp_<true>(o.sh * o.sh * o.sh);
       ss.lg = log10_(o.sh);
+    } else if constexpr (KIND == KERR_SYNCH) {
+      // The colour slopes are -2^E with E = K_c + P - x3_c log2(e) d + 2 log2(x3 + 2^(11/12))
+      // (colour_f32): the stage part P = log2(n / n_0) - 11/3 log2(shift) and
+      // d = shift^(1/3) - 1, in double so that d keeps its relative accuracy (the
+      // slopes scale as shift^(-x3/3), x3 ~ 100, at optical frequencies)
+      ss.P = __builtin_amdgcn_logf(o.w) - (11.0f / 3.0f) * __builtin_amdgcn_logf(o.sh);
+      ss.lg = (float)(cbrt((double)o.sh) - 1.0);
     } else {
       ss.P = q * float(S.bb_a);
     }
@@ -202,6 +214,16 @@ __device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, 
     T base = T(ss.P) * T(c.if3) * g;
     kT = (double)(base * (T(1) - em));
     kI = (double)(base * ES * exp_(-u - T(tau)));
+  } else if constexpr (KIND == KERR_SYNCH) {
+    // thermal synchrotron with Kirchhoff's law, in double for either T:
+    // a = Q n w expm1(h nu / k T_e), j / nu^3 = ES Q n w, w = G(X) / X^3
+    (void)ES;
+    const double x3 = c.if3 * ss.lg;  // X^(1/3) = X_c^(1/3) shift^(1/3)
+    const double x32 = x3 * x3, x34 = x32 * x32;
+    const double w = sq(x3 + SYN_2_11_12) * exp(-x3) / (x34 * x34);
+    const double base = ss.P * w;
+    kT = base * expm1(c.uf * ss.sh);
+    kI = base * EMISSION_SCALE * exp(-tau);
   } else {
     // a = const, j = a B_nu(T)
     T P = T(ss.P);
@@ -399,6 +421,20 @@ struct Integrator {
         const float base = ss.P * if3 * g;
         kTm = base * (1.0f - __expf(-u));
         kIm = base * ES * __expf(-u - (float)tau);
+      } else if constexpr (KIND == KERR_SYNCH) {
+        // log2 form of Q n w / shift with x3 = x3_c (1 + d): the large constants
+        // (log2 Q ~ 190, x3_c log2(e) ~ 150, 8 log2 x3_c ~ 50) are folded into the
+        // colour constant K_c = lc in double, so FP32 only adds terms of a few tens.
+        // Exponents are clamped at 120 so that FP32 stays finite where FP64 would
+        // hold a huge but finite slope (the step is rejected either way); the clamp
+        // lets a NaN through, as the FP64 form would.
+        const float x3c = if3;
+        const float x3 = fmaf(x3c, ss.lg, x3c);
+        const float E = lc + ss.P - (x3c * 1.4426950408889634f) * ss.lg +
+                        2.0f * __builtin_amdgcn_logf(x3 + float(SYN_2_11_12));
+        const float EI = E - (float)tau * 1.4426950408889634f;
+        kTm = -__builtin_amdgcn_exp2f(E > 120.0f ? 120.0f : E) * expm1_(uf * ss.sh);
+        kIm = -ES * __builtin_amdgcn_exp2f(EI > 120.0f ? 120.0f : EI);
       } else {
         kTm = ss.P;
         kIm = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);

@@ -1,8 +1,8 @@
 """Generate the committed fixtures under tests/golden/ from the oracle.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--flops-only | --synch-only]
 
-* oracle_<cfg>_<npix>px_<nlam>.npz: small renders of the benchmark workloads
+* oracle_<cfg>_<npix>px_<nlam>[_<n>poses].npz: small renders of the benchmark workloads
   (XYZ, class, attempted/accepted steps, final states) -- the golden vectors that the
   GPU parity tests compare against and that the CPU tests use to detect any change
   in the oracle;
@@ -29,6 +29,10 @@ import oracle as O  # noqa: E402
 from rtwca_amd import scenes  # noqa: E402
 
 FIXTURES = [(1, 16, 64), (2, 16, 16), (3, 16, 64), (3, 12, 8), (4, 12, 16)]
+# synchrotron torus + jet: four of the 64 poses (equator, top of the orbit, an oblique
+# pose and one below the equator), rays of all four concatenated in pose order
+SYNCH_POSES = (0, 16, 27, 40)
+SYNCH_FIXTURE = (5, 10, 16)
 
 
 def fixture_name(idx, x_pix, nlam):
@@ -44,15 +48,31 @@ def make_fixtures():
         print("wrote", fixture_name(idx, x_pix, nlam), "steps", int(r["steps"][:, 0].sum()))
 
 
+def synch_fixture_name():
+    idx, x_pix, nlam = SYNCH_FIXTURE
+    return os.path.join(HERE, "oracle_cfg%d_%dpx_%dl_%dposes.npz" % (idx, x_pix, nlam, len(SYNCH_POSES)))
+
+
+def make_synch_fixture():
+    idx, x_pix, nlam = SYNCH_FIXTURE
+    w = scenes.config(idx, x_pix=x_pix, n_lambda=nlam)
+    parts = [O.render(w.scene, w.cameras[k], w.integ, w.lam, nthreads=os.cpu_count() or 1, want_rays=True)
+             for k in SYNCH_POSES]
+    cat = {key: np.concatenate([p[key] for p in parts]) for key in ("xyz", "cls", "steps", "rays")}
+    np.savez_compressed(synch_fixture_name(), poses=np.array(SYNCH_POSES), lam=np.array(w.lam), **cat)
+    print("wrote", synch_fixture_name(), "steps", int(cat["steps"][:, 0].sum()))
+
+
 def make_flops():
     out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
                          "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
-    for idx, x_pix in ((1, 64), (2, 64), (3, 64), (4, 64)):
+    for idx, x_pix in ((1, 64), (2, 64), (3, 64), (4, 64), (5, 64)):
         w = scenes.config(idx, x_pix=x_pix)
         npix = x_pix * x_pix
         sample = list(range(0, npix, max(1, npix // 96)))
         c = O.count_ops(w.scene, w.cameras[0], w.integ, w.lam, sample)
         nv = c["steps"] - c["steps_transfer"]
+        cg = O.count_ops(w.scene, w.cameras[0], w.integ, [], sample)  # geodesic only (no colours)
         out["cfg%d" % idx] = {"workload": w.name, "arith_per_step": c["arith"] / c["steps"],
                               "trans_per_step": c["trans"] / c["steps"],
                               "flops_per_step": (c["arith"] + c["trans"]) / c["steps"],
@@ -61,6 +81,9 @@ def make_flops():
                                                          max(1.0, c["steps_transfer"]),
                               "flops_per_vacuum_step": (c["arith"] + c["trans"] - c["arith_transfer"] -
                                                         c["trans_transfer"]) / max(1.0, nv),
+                              # a step of the 8 geodesic components alone: the work of a step whose
+                              # transfer components are frozen (the vacuum kernel's steps)
+                              "flops_per_geodesic_step": (cg["arith"] + cg["trans"]) / cg["steps"],
                               "sample_rays": len(sample), "sample_steps": int(c["steps"]),
                               "sample_transfer_steps": int(c["steps_transfer"])}
         print(idx, out["cfg%d" % idx])
@@ -69,6 +92,9 @@ def make_flops():
 
 
 if __name__ == "__main__":
-    if "--flops-only" not in sys.argv:
+    if "--synch-only" in sys.argv:
+        make_synch_fixture()
+    elif "--flops-only" not in sys.argv:
         make_fixtures()
+        make_synch_fixture()
     make_flops()
