@@ -49,8 +49,20 @@ enum rtw_scene_kind {
   RTW_SCENE_SCHWARZSCHILD_DISK = 2, /* main/Schwarzschild_Simple_Disk_Spherical.source */
   RTW_SCENE_KERR_DISK = 3,          /* main/Kerr.source (spin a; Powehi geometry at a = 0.9) */
   RTW_SCENE_KERR_THICK_FF = 4,      /* Kerr + Gaussian thick disk, free-free with van Hoof Gaunt */
-  RTW_SCENE_KERR_SYNCH = 5          /* Kerr + thermal-synchrotron torus and conical jet */
+  RTW_SCENE_KERR_SYNCH = 5,         /* Kerr + thermal-synchrotron torus and conical jet */
+  RTW_SCENE_POWEHI_M87 = 6          /* main/Powehi.source: the M87* model */
 };
+
+/* RTW_SCENE_POWEHI_M87 follows main/Powehi.source: Kerr geometry at spin 0.9f0
+ * (set RTW_SCENE_REF_F32 for the reference's Float32 evaluation); a wedge disk
+ * r_in <= r <= r_out, th_lo <= theta <= th_hi (the reference: the formula ISCO,
+ * 12, 31pi/64, 33pi/64) radiating as a blackbody at `temperature` with a_nu =
+ * absorb (source_model.jl:190-194, 231); a conical jet r >= r_jet (10) outside
+ * 5..175 degrees (:233-238), rotating at min(0.9 / (r sin theta), Omega_jet) and
+ * accelerating from Lorentz factor 1.2 (:287-301), with power-law synchrotron
+ * (p = 2.5, Marszewski 2021) and the Tsunetoe (2025) anisotropy in the field
+ * B^phi = bfield Gauss (:51-175, :196-218).  The file's other constants are
+ * fixed as there. */
 
 /* RTW_SCENE_KERR_SYNCH (no reference counterpart; the reference's nearest scene
  * is main/Powehi.source with power-law synchrotron in a 5-degree jet):

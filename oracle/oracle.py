@@ -58,6 +58,8 @@ def lib():
         L.rtwo_synch_derived.argtypes = [P, P]
         L.rtwo_synch_point.argtypes = [P, P, P]
         L.rtwo_radiative.argtypes = [P, P, D, P]
+        L.rtwo_fluid.argtypes = [P, P, P, P]
+        L.rtwo_powehi_consts.argtypes = [P, P]
         rc = L.rtwo_load_gaunt(GAUNT_PATH.encode())
         if rc != 0:
             raise RuntimeError("oracle: cannot load Gaunt table (%d)" % rc)
@@ -109,10 +111,26 @@ def synch_point(scene, x):
 
 
 def radiative(scene, x, nu):
-    """(a_nu, j_nu) at x for fluid-frame frequency nu (nm^-1)."""
+    """(a_nu, j_nu) at x for fluid-frame frequency nu (nm^-1); x is a position (k taken as
+    0) or a ray state of 8 components."""
+    x8 = np.zeros(8)
+    x8[:len(x)] = x
     out = np.zeros(2)
-    lib().rtwo_radiative(_r(scene), _p(np.ascontiguousarray(x, dtype=np.float64)), float(nu), _p(out))
+    lib().rtwo_radiative(_r(scene), _p(x8), float(nu), _p(out))
     return out[0], out[1]
+
+
+def fluid(scene, ray8):
+    """(fluid_params[8], source velocity[4]) at a ray state."""
+    fp, u = np.zeros(8), np.zeros(4)
+    lib().rtwo_fluid(_r(scene), _p(np.ascontiguousarray(ray8, dtype=np.float64)), _p(fp), _p(u))
+    return fp, u
+
+
+def powehi_consts(scene):
+    out = np.zeros(3)
+    lib().rtwo_powehi_consts(_r(scene), _p(out))
+    return dict(zip(("approx_p_synch_2_5", "omega_jet", "nozzle_acc"), out))
 
 
 def metric(scene, x):

@@ -136,6 +136,7 @@ enum SceneKind {
   KERR_DISK = 3,          // Kerr.source (and Powehi geometry with a = 0.9)
   KERR_THICK_FF = 4,      // new: Kerr geometry + Gaussian thick disk, free-free with van Hoof Gaunt
   KERR_SYNCH = 5,         // new: Kerr geometry + thermal-synchrotron torus and conical jet
+  POWEHI_M87 = 6,         // Powehi.source: the M87* model (wedge disk + power-law synchrotron jet)
 };
 enum SceneFlags {
   REF_F32 = 1,  // Kerr family: Float32 Christoffel, horizon/ISCO and max_dt_scale as the reference
@@ -549,6 +550,40 @@ template <class R> int synch_region(const Scene& s, const R* x, R& dens) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Powehi.source (the reference's M87* model), source_model.jl.  Its geometry is
+// the Kerr file with alpha = 0.9f0 (REF_F32).  Julia evaluates with the ray in
+// Float64 and the Float32 constants promoted, which is what these functions do:
+// each Float32 constant is taken as (double)float.  The disk wedge, its a_nu and
+// temperature and the jet base radius come from the scene (r_in/r_out/th_lo/th_hi,
+// absorb, temperature, r_jet); the rest are the file's constants.
+constexpr float PW_MAP_SCALE = 9.62e12f;                               // :6
+inline float pw_synch_em() { return (float)(7.694e-37 * (double)PW_MAP_SCALE * 2.8e7); }  // :8
+inline float pw_synch_ab() { return (float)(2.8173e-24 * (double)PW_MAP_SCALE); }         // :10
+constexpr float PW_NE0 = 1e7f;                                         // :12
+constexpr float PW_GAMMA_MIN = 1.2f, PW_GAMMA_MAX = 1e1f, PW_R_MAX_NOZZLE = 75.0f;  // :16-19
+constexpr float PW_C_FIT = +7.951834e-01f, PW_D_FIT = +9.832989e-01f, PW_E_FIT = -5.752283e-01f;  // :25-27
+constexpr float PW_JET_TH_LO = 0.087263889f, PW_JET_TH_HI = 3.054236111f;  // :233-234
+constexpr float PW_POLE_LO = 1e-3f, PW_POLE_HI = 3.1406f;
+// Omega_jet = 0.5 alpha / (r_s r_ev) and the nozzle acceleration, in Float32 (:15, :20)
+inline float pw_omega_jet(const Scene& s) { return 0.5f * (float)s.spin / ((float)s.r_s * (float)s.r_ev); }
+inline float pw_nozzle_acc(const Scene& s) {
+  return (PW_GAMMA_MAX - PW_GAMMA_MIN) / (PW_R_MAX_NOZZLE - (float)s.r_jet);
+}
+// Approx_P_synch (:28-30)
+template <class R> R pw_approx_p_synch(R p) {
+  return exp(R((double)PW_C_FIT) + R((double)PW_E_FIT) * p) * pow(p, R((double)PW_D_FIT) * p);
+}
+// region_calc (:229-243): 1 wedge disk, 2 jet cone, 3 empty
+template <class R> int pw_region(const Scene& s, const R* x) {
+  if (R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && R(s.th_lo) <= x[2] && x[2] <= R(s.th_hi)) return 1;
+  if (x[1] >= R((double)(float)s.r_jet) &&
+      !(R((double)PW_JET_TH_LO) <= x[2] && x[2] <= R((double)PW_JET_TH_HI)) &&
+      (R((double)PW_POLE_LO) <= x[2] && x[2] <= R((double)PW_POLE_HI)))
+    return 2;
+  return 3;
+}
+
 // Matter region test (is_fire).  Minkowski source_model.jl:253-260 (r <= 1),
 // Schwarzschild :273-280 and Kerr :73-80 (wedges); the thick disk (new) is a
 // radial band with |cos theta| <= z_cut * h_over_r.
@@ -564,6 +599,7 @@ template <class R> bool is_fire(const Scene& s, const R* x) {
       R d;
       return synch_region(s, x, d) != 0;
     }
+    case POWEHI_M87: return pw_region(s, x) != 3;
     default: return false;
   }
 }
@@ -626,7 +662,24 @@ template <class R> void source_velocity(const Scene& s, const R* x, const R g[4]
     R d;
     if (synch_velocity(s, x, synch_region(s, x, d), u)) return;
   }
-  bool keplerian = (s.kind == KERR_SYNCH) ? false : is_fire(s, x);
+  if (s.kind == POWEHI_M87 && pw_region(s, x) == 2) {
+    // rotating, accelerating conical jet (Powehi.source/source_model.jl:271-292)
+    const R w = -g[0][3] / (g[3][3] + R(NO_DIV_ZERO));
+    const R Om = rmin(R((double)0.9f) * (R(1) / sin(x[2])) / x[1], R((double)pw_omega_jet(s)));
+    const R Sig = sq(x[1]) + sq(a * cos(x[2]));
+    const R Del = sq(x[1]) - x[1] * R(s.r_s) + sq(a);
+    const R A = (sq(x[1]) + sq(a)) * Sig + R(s.r_s) * x[1] * sq(a * sin(x[2]));
+    const R inv_lapse = sqrt(A / (Sig * Del));
+    const R G = R((double)PW_GAMMA_MIN) +
+                rmin(R((double)pw_nozzle_acc(s)) * (x[1] - R((double)(float)s.r_jet)), R((double)PW_GAMMA_MAX));
+    const R ut = G * inv_lapse;
+    u[0] = ut;
+    u[1] = sqrt((sq(G) - R(1) - g[3][3] * sq(G * inv_lapse) * sq(Om - w)) / g[1][1]);
+    u[2] = R(0);
+    u[3] = ut * Om;
+    return;
+  }
+  bool keplerian = (s.kind == KERR_SYNCH) ? false : (s.kind == POWEHI_M87 ? pw_region(s, x) == 1 : is_fire(s, x));
   if (keplerian) {
     R Om = sqrt(M) / (pow(x[1], R(1.5)) + a * sqrt(M));
     R D = pow(x[1], R(1.5)) - R(3) * M * sqrt(x[1]) + R(2) * a * sqrt(M);
@@ -662,6 +715,52 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
       fp[1] = R(0);
     }
   }
+  if (s.kind == POWEHI_M87) {  // Powehi.source/source_model.jl:154-175
+    fp[0] = R(2.5);   // power-law index p
+    fp[1] = R(30.0);  // gamma_min
+    fp[2] = R(1e9);   // gamma_max
+    // calc_B_field (:51-57): (B^r, B^theta, B^phi) = (0, 0, bfield) Gauss
+    const R B2(0.0), B3(0.0), B4((double)(float)s.bfield);
+    // transform_B_b (:63-114)
+    R ps(0.0), b1(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][1] * u[i];
+    b1 += B2 * ps;
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][2] * u[i];
+    b1 += B3 * ps;
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][3] * u[i];
+    b1 += B4 * ps;
+    const R b2 = (R(1) / u[0]) * (B2 + b1 * u[1]);
+    const R b3 = (R(1) / u[0]) * (B3 + b1 * u[2]);
+    const R b4 = (R(1) / u[0]) * (B4 + b1 * u[3]);
+    // "b^2" as the reference forms it: sum_mu (sum_i g_i,mu) (b^mu)^2 (quirk: a column sum
+    // of the metric, not the quadratic form g_mu,nu b^mu b^nu)
+    R bm(0.0);
+    for (int i = 0; i < 4; ++i) bm += g[i][0] * sq(b1);
+    for (int i = 0; i < 4; ++i) bm += g[i][1] * sq(b2);
+    for (int i = 0; i < 4; ++i) bm += g[i][2] * sq(b3);
+    for (int i = 0; i < 4; ++i) bm += g[i][3] * sq(b4);
+    bm = rmax(R((double)1e-12f), bm);
+    fp[3] = R((double)PW_NE0) * bm;
+    // calc_b_angle (:118-149)
+    R bk(0.0);
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][0] * x[i + 4];
+    bk += b1 * ps;
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][1] * x[i + 4];
+    bk += b2 * ps;
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][2] * x[i + 4];
+    bk += b3 * ps;
+    ps = R(0.0);
+    for (int i = 0; i < 4; ++i) ps += g[i][3] * x[i + 4];
+    bk += b4 * ps;
+    fp[4] = rmin(sq(bk) / (bm * sq(fs)), R(1.0)) + R(NO_DIV_ZERO);
+    fp[5] = R((double)9.34e-11f) * sqrt(bm);  // cyclotron frequency in nm^-1
+    fp[6] = R(s.temperature);                 // disk blackbody temperature
+  }
   if (s.kind == KERR_SYNCH) {  // electron temperature, n_e / n_0, region
     R d;
     fp[2] = R((double)synch_region(s, x, d));
@@ -687,6 +786,33 @@ template <class R> void radiative_coefficients(const Scene& s, const R* x, const
       if (std::is_same<R, Cnt>::value) { op_count().arith += 14; op_count().trans += 2; }
       a_nu = R(s.absorb) * fp[1] * sqrt(R(FF_T0) / T) * cube(R(FF_NU0) / nu) * (R(1) - exp(-u)) * R((double)gff);
       j_nu = a_nu * planck(T, nu);
+    } else {
+      a_nu = R(0); j_nu = R(0);
+    }
+    return;
+  }
+  if (s.kind == POWEHI_M87) {  // Powehi.source/source_model.jl:186-223
+    const int zone = pw_region(s, x);
+    if (zone == 1) {
+      a_nu = R((double)(float)s.absorb);
+      j_nu = a_nu * planck(fp[6], nu);
+    } else if (zone == 2) {
+      const R p = fp[0], gmin = fp[1], gmax = fp[2], c2 = fp[4];
+      const R st = sqrt(rmax(R((double)1e-12f), R(1) - c2));
+      j_nu = R((double)(PW_NE0 * pw_synch_em())) * (fp[5] / R((double)9.34e-11f));
+      a_nu = R((double)(PW_NE0 * pw_synch_ab())) / nu;
+      const R gd = pow(gmin, R(1) - p) - pow(gmax, R(1) - p);
+      j_nu = j_nu * (pow(R(3), p / R(2)) * (p - R(1)) * st / (R(2) * (p + R(1)) * gd));
+      const double g1 = std::tgamma(val((R(3) * p - R(1)) / R(12))), g2 = std::tgamma(val((R(3) * p + R(19)) / R(12)));
+      j_nu = j_nu * (R(g1) * R(g2) * pow(nu / (fp[5] * st), -(p - R(1)) / R(2)));
+      a_nu = a_nu * (pow(R(3), (p + R(1)) / R(2)) * (p - R(1)) / (R(4) * gd));
+      const double g3 = std::tgamma(val((R(3) * p + R(2)) / R(12))), g4 = std::tgamma(val((R(3) * p + R(22)) / R(12)));
+      if (std::is_same<R, Cnt>::value) op_count().trans += 4;
+      a_nu = a_nu * (R(g3) * R(g4) * pow(nu / (fp[5] * st), -(p + R(2)) / R(2)));
+      // anisotropic distribution (Tsunetoe 2025)
+      const R Phi = (R(1) / pw_approx_p_synch(p)) * pow(R(1) - R((double)0.99f) * c2, -p / R(2));
+      j_nu = j_nu * Phi;
+      a_nu = a_nu * Phi;
     } else {
       a_nu = R(0); j_nu = R(0);
     }

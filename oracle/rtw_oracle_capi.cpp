@@ -129,8 +129,8 @@ void rtwo_synch_point(const rtw_scene* s, const double* x, double* out3) {
   out3[1] = d;
   out3[2] = synch_lnh(sc, x);
 }
-// Absorption and emission coefficients (a_nu, j_nu) at x for the fluid-frame
-// frequency nu (nm^-1), with the scene's fluid parameters at x.
+// Absorption and emission coefficients (a_nu, j_nu) at the ray state x[8] for the
+// fluid-frame frequency nu (nm^-1), with the scene's fluid parameters there.
 void rtwo_radiative(const rtw_scene* s, const double* x, double nu, double* out2) {
   Scene sc = to_scene(s);
   double g[4][4], u[4], fp[8] = {0};
@@ -138,8 +138,29 @@ void rtwo_radiative(const rtw_scene* s, const double* x, double nu, double* out2
     for (int j = 0; j < 4; ++j) g[i][j] = (i == j);
   lower_metric<double>(sc, x, g);
   source_velocity<double>(sc, x, g, u);
-  fluid_params<double>(sc, x, u, g, 1.0, fp);
+  fluid_params<double>(sc, x, u, g, freq_shift<double>(x, u, g), fp);
   radiative_coefficients<double>(sc, x, fp, nu, out2[0], out2[1]);
+}
+
+// Fluid parameters at a ray state (calc_fluid_params!, after the velocity and the
+// frequency shift as in calc_ray_derivative!); out8 = fluid_params, u4 = velocity.
+void rtwo_fluid(const rtw_scene* s, const double* ray8, double* out8, double* u4) {
+  Scene sc = to_scene(s);
+  double g[4][4], fp[8] = {0};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) g[i][j] = (i == j);
+  lower_metric<double>(sc, ray8, g);
+  source_velocity<double>(sc, ray8, g, u4);
+  const double fs = freq_shift<double>(ray8, u4, g);
+  fluid_params<double>(sc, ray8, u4, g, fs, fp);
+  for (int i = 0; i < 8; ++i) out8[i] = fp[i];
+}
+// Powehi constants: Approx_P_synch(2.5), Omega_jet, nozzle acceleration.
+void rtwo_powehi_consts(const rtw_scene* s, double* out3) {
+  Scene sc = to_scene(s);
+  out3[0] = pw_approx_p_synch<double>(2.5);
+  out3[1] = pw_omega_jet(sc);
+  out3[2] = pw_nozzle_acc(sc);
 }
 
 void rtwo_source_velocity(const rtw_scene* s, const double* x, double* u4) {

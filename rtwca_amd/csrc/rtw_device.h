@@ -27,7 +27,7 @@ constexpr double GAM2_FACTOR = 2.1798723611035e-18 / K_B;         // Ry / k_B
 constexpr double FF_T0 = 1e4;           // thick disk opacity normalisation temperature
 constexpr double FF_NU0 = 1.0 / 550.0;  // and frequency (nm^-1)
 
-enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5 };
+enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5, POWEHI = 6 };
 constexpr int SCENE_GAUNT_FP64 = 2;  // Gaunt interpolation in FP64 (scene flag)
 enum { CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
 
@@ -54,6 +54,11 @@ struct DevScene {
   // jet base radius, cos(half-angle) and 1 / (1 - cos(half-angle))
   double sy_ell, sy_Fin, sy_ihm, sy_Q, sy_l2Q, sy_rjet, sy_cjet, sy_iwj;
   double sy_nus;  // synchrotron frequency nu_s (Hz), host side only
+  // M87* (Powehi.source/source_model.jl) jet: Omega_jet, nozzle acceleration and
+  // base radius (Float32 constants of the file); B^phi; the per-scene factors of
+  // j and a (power law p = 2.5 with its Gamma functions and 1 / Approx_P_synch(p));
+  // the exponents -(p-1)/2, -(p+2)/2 and -p/2
+  double pw_omj, pw_noz, pw_rjet, pw_bphi, pw_cj, pw_ca, pw_ej, pw_ea, pw_ephi;
 };
 
 struct DevOpts {
@@ -65,7 +70,7 @@ struct DevOpts {
 struct DevColor {
   double f;     // 1 / lambda (nm^-1)
   double uf;    // F_FACTOR f / T  (u = uf * shift)
-  double lc;    // free-free: log10(uf); synchrotron: K = log2 Q - x3 log2(e) - 8 log2 x3
+  double lc;    // free-free: log10(uf); synchrotron: K = log2 Q - x3 log2(e) - 8 log2 x3; M87*: log2 f
   double if3;   // free-free: (nu0 / f)^3; synchrotron: x3 = (nu / nu_s)^(1/3) at unit shift
   double cx;    // f^3 c / lambda^2: I_lambda = (I/nu^3) * cx  (color_theory.jl:73)
   double wx, wy, wz;  // CIE weights (color_theory.jl:79-91)
@@ -175,7 +180,9 @@ template <bool FAST> __device__ __forceinline__ double ctrl_factor(double err) {
 template <class T> struct StageOut {
   T d[8];      // dx/dlambda
   T sh;        // frequency shift -u_j g_ij k^i (gen_geometry.jl:373-390)
-  T w;         // thick disk: rho^2 at the point; synchrotron: n_e / n_0
+  T w;         // thick disk: rho^2 at the point; synchrotron: n_e / n_0; M87* jet: j scale J0
+  T w2, w3;    // M87* jet: absorption scale A0 and 1 / (nu_c sin(theta))
+  int zone;    // M87*: 1 disk, 2 jet, 3 empty
   bool matter; // is_fire
 };
 
@@ -326,7 +333,59 @@ __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T 
     T gtp = -rs * a * r * s2 * iSg;
     T gpp = A * iSg * s2;
     bool m;
-    if constexpr (KIND == KERR_SYNCH) {
+    if constexpr (KIND == POWEHI) {
+      // region_calc (Powehi.source/source_model.jl:229-243)
+      const bool disk = T(S.r_in) <= r && r <= T(S.r_out) && T(S.th_lo) <= th && th <= T(S.th_hi);
+      const bool jet = !disk && r >= T(S.pw_rjet) && !(T(0.087263889f) <= th && th <= T(3.054236111f)) &&
+                       (T(1e-3f) <= th && th <= T(3.1406f));
+      o.zone = disk ? 1 : (jet ? 2 : 3);
+      o.matter = disk || jet;
+      const T grr = Sg * rcp_<F>(D), gthth = Sg;
+      T ut, up, ur = T(0);
+      if (disk) {  // prograde Keplerian orbit (:276-286)
+        const T sr = sqrt_<F>(r), r15 = r * sr, sM = sqrt_<F>(M);
+        const T Om = sM * rcp_<F>(r15 + a * sM);
+        const T Dk = r15 - T(3) * M * sr + T(2) * a * sM;
+        ut = (r15 + a * sM) * rcp_<F>(sqrt_<F>(r15) * sqrt_<F>(Dk));
+        up = ut * Om;
+      } else if (jet) {  // rotating, accelerating conical jet (:287-301)
+        const T w = -gtp * rcp_<F>(gpp + T(NO_DIV_ZERO));
+        const T Om0 = T(0.9f) * rcp_<F>(s) * rcp_<F>(r), Omj = T(S.pw_omj);
+        const T Om = (Omj < Om0) ? Omj : Om0;
+        const T ia = sqrt_<F>(A * rcp_<F>(Sg * D));
+        const T dG = T(S.pw_noz) * (r - T(S.pw_rjet));
+        const T G = T(1.2f) + ((T(10.0f) < dG) ? T(10.0f) : dG);
+        ut = G * ia;
+        ur = sqrt_<F>((G * G - T(1) - gpp * ((G * ia) * (G * ia)) * ((Om - w) * (Om - w))) * rcp_<F>(grr));
+        up = ut * Om;
+      } else {  // FIDO (:302-316)
+        ut = sqrt_<F>(A * iSg * rcp_<F>(D));
+        up = ut * (T(2) * a * M * r * rcp_<F>(A));
+      }
+      o.sh = -(ut * (gtt * kt + gtp * kph) + up * (gtp * kt + gpp * kph) + ur * (grr * kr));
+      if (jet) {
+        // calc_fluid_params! (:154-175): B = (0, 0, B^phi) -> fluid-frame b (transform_B_b,
+        // :63-114, with the reference's column-sum "b^2"), pitch angle (calc_b_angle, :118-149).
+        // The zero B^r, B^theta terms are kept so that a NaN u^r propagates as in the reference.
+        const T B2 = T(0), B3 = T(0), B4 = T(S.pw_bphi);
+        const T b1 = B2 * (grr * ur) + B3 * (gthth * T(0)) + B4 * (gtp * ut + gpp * up);
+        const T iut = rcp_<F>(ut);
+        const T b2 = iut * (B2 + b1 * ur), b3 = iut * (B3 + b1 * T(0)), b4 = iut * (B4 + b1 * up);
+        T bm = (gtt + gtp) * (b1 * b1) + grr * (b2 * b2) + gthth * (b3 * b3) + (gtp + gpp) * (b4 * b4);
+        bm = (bm < T(1e-12f)) ? T(1e-12f) : bm;
+        const T bk = b1 * (gtt * kt + gtp * kph) + b2 * (grr * kr) + b3 * (gthth * kth) + b4 * (gtp * kt + gpp * kph);
+        T c2 = bk * bk * rcp_<F>(bm * o.sh * o.sh);
+        c2 = ((c2 < T(1)) ? c2 : T(1)) + T(NO_DIV_ZERO);
+        const T sb = sqrt_<F>(bm);
+        const T st = sqrt_<F>((T(1) - c2 > T(1e-12f)) ? T(1) - c2 : T(1e-12f));
+        // anisotropy (1 - 0.99 cos^2)^(-p/2); the 1 / Approx_P_synch(p) is in the scene factors
+        const T phi = exp_(T(S.pw_ephi) * log_(T(1) - T(0.99f) * c2));
+        o.w = T(S.pw_cj) * sb * st * phi;  // j_nu = J0 (nu / (nu_c sin))^(-(p-1)/2)
+        o.w2 = T(S.pw_ca) * phi;           // a_nu = A0 / nu (nu / (nu_c sin))^(-(p+2)/2)
+        o.w3 = rcp_<F>(T(9.34e-11f) * sb * st);
+      }
+      return;
+    } else if constexpr (KIND == KERR_SYNCH) {
       // Fishbone-Moncrief torus: ln h = F(r, th) - F(r_in, pi/2),
       // F = 1/2 ln[(1 + q) A / (Sigma Delta)] - q/2 - 2 a M r l / A,
       // q = sqrt(1 + 4 s^2), s^2 = l^2 Sigma^2 Delta / (A^2 sin^2 th); then the jet
@@ -597,13 +656,17 @@ __device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, i
 // Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
 struct StageSc {
   double sh;   // shift
-  double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4; synchrotron: -Q n / shift
-  double lg;   // free-free: log10(shift); synchrotron: shift^(1/3)
+  double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4; synchrotron: -Q n / shift;
+               // M87* jet: -J0 / shift
+  double lg;   // free-free: log10(shift); synchrotron: shift^(1/3); M87* jet: -A0 / shift
   double vac;  // 1 if no matter at this stage
+  double q;    // M87*: 1 / (nu_c sin(theta)) in the jet, 0 in the disk
 };
 // The same in FP32, for the mixed-precision stages.
 struct StageF {
   float sh, P, lg, vac;  // synchrotron: P = log2(n / n_0) - 11/3 log2(shift), lg = shift^(1/3) - 1
+  float q;               // M87* jet: P = -J0 / shift^4, lg = -A0 / shift^2, q = log2(shift / (nu_c sin));
+                         // disk: q = -inf (blackbody P as the other scenes)
 };
 
 }  // namespace rtw

@@ -141,6 +141,25 @@ def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, 
     return s
 
 
+def powehi_m87(ref_f32=True):
+    """main/Powehi.source, the reference's M87* model: Kerr geometry with alpha = 0.9f0
+    (source_geometry.jl:12); a wedge disk from the reference-formula ISCO to r = 12,
+    31pi/64 <= theta <= 33pi/64, blackbody at 5778 K with a_nu = 1f-2
+    (source_model.jl:172,190-194,231); a conical jet r >= 10 outside 5..175 degrees
+    with power-law synchrotron in B^phi = 10 G (:51-57, :233-238)."""
+    s = _base(abi.SCENE_POWEHI_M87)
+    s.flags = abi.SCENE_REF_F32 if ref_f32 else 0
+    s.spin = f32(0.9) if ref_f32 else 0.9
+    s.temperature = 5778.0
+    s.absorb = f32(1e-2)
+    s.r_in = kerr_isco_ref(0.9, 1.0, ref_f32)
+    s.r_out = 12.0
+    s.th_lo, s.th_hi = 31 * PI / 64, 33 * PI / 64
+    s.bfield = 10.0
+    s.r_jet = 10.0
+    return s
+
+
 def kerr_synch(spin=0.94, r_in=6.0, r_max=12.0, density=1e45, jet_ratio=1e-4, jet_half_angle_deg=10.0,
                r_jet=10.0, bfield=10.0, theta_e=10.0):
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Fishbone-Moncrief
@@ -196,6 +215,8 @@ def initialize_world(name):
         return schwarzschild_disk(), [0, 25, PI / 3, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     if n == "kerr_boyer_lindquist":
         return kerr_disk(), [0, 25, 0.52 * PI, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
+    if n == "m87":  # initialization.jl:100-108
+        return powehi_m87(), [0, 35, 0.3, 0], [PI / 2, PI / 6], [0.0, 0.0, PI], 0.0
     if n == "minkowski_t,r,θ,ϕ_sphere":
         return minkowski_sphere(), [0, 4, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     raise ValueError("scene %r is not provided by rtwca_amd" % name)

@@ -1,6 +1,6 @@
 """Generate the committed fixtures under tests/golden/ from the oracle.
 
-    python tests/golden/make_golden.py [--flops-only | --synch-only]
+    python tests/golden/make_golden.py [--flops-only | --synch-only | --m87-only]
 
 * oracle_<cfg>_<npix>px_<nlam>[_<n>poses].npz: small renders of the benchmark workloads
   (XYZ, class, attempted/accepted steps, final states) -- the golden vectors that the
@@ -63,6 +63,31 @@ def make_synch_fixture():
     print("wrote", synch_fixture_name(), "steps", int(cat["steps"][:, 0].sum()))
 
 
+M87_FIXTURE = (16, 16)  # x_pix (FOV 85 x 60, the gen_image defaults), wavelengths
+
+
+def m87_workload(x_pix=16, nlam=16):
+    """The reference's "m87" preset (initialization.jl:100-108, main/Powehi.source) with
+    gen_image's default FOV and the tolerance of its timing runs (1e-7)."""
+    from rtwca_amd import abi
+    sc, pos, d, p, beta = scenes.initialize_world("m87")
+    cam = abi.make_camera(pos, d, beta, p, x_pix, 85, 60)
+    it = abi.make_integrator(1e-7, 1e-7, 0.1, 1e4)
+    return sc, cam, it, scenes.wavelengths(380, 780, nlam)
+
+
+def m87_fixture_name():
+    return os.path.join(HERE, "oracle_m87_%dpx_%dl.npz" % M87_FIXTURE)
+
+
+def make_m87_fixture():
+    sc, cam, it, lam = m87_workload(*M87_FIXTURE)
+    r = O.render(sc, cam, it, lam, nthreads=os.cpu_count() or 1, want_rays=True)
+    np.savez_compressed(m87_fixture_name(), xyz=r["xyz"], cls=r["cls"], steps=r["steps"], rays=r["rays"],
+                        lam=np.array(lam))
+    print("wrote", m87_fixture_name(), "steps", int(r["steps"][:, 0].sum()))
+
+
 def make_flops():
     out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
                          "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
@@ -94,7 +119,11 @@ def make_flops():
 if __name__ == "__main__":
     if "--synch-only" in sys.argv:
         make_synch_fixture()
+    elif "--m87-only" in sys.argv:
+        make_m87_fixture()
+        sys.exit(0)
     elif "--flops-only" not in sys.argv:
         make_fixtures()
         make_synch_fixture()
+        make_m87_fixture()
     make_flops()
