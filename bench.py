@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--x-pix 1024] [--no-cpu]
     python bench.py --workload synch [--poses 64] [--x-pix 512]   # batched synchrotron scene
+    python bench.py --workload reference-suite                     # the reference's timed tests
 
 The second form times the 64-pose thermal-synchrotron torus + jet batch (64 cameras
 along an inclined orbit at r = 50, 512x512 each, one render call, the same
@@ -44,7 +45,20 @@ PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32 (AMD published)
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
 TRAFFIC_SUMMARY = {3: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_1024_mixed_hbm.md"),
                    5: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_synch_64x512_mixed_hbm.md")}
-WORKLOADS = {"kerr": 3, "synch": 5}
+WORKLOADS = {"kerr": 3, "synch": 5, "reference-suite": 0}
+
+# The reference's own timed tests (main/initialization.jl:352-512: 500 px wide, FOV 85 x
+# 60, 350:30:740 nm, max_dt_scale 0.1, max_steps 1e4) and the times it recorded
+# (main/Test Results/TestResults.csv; "5P" = CUDA.jl Float32 on an unstated NVIDIA GPU,
+# "LttM" = Float64 on the CPU with all threads).
+REFERENCE_SUITE = [
+    # name, initialize_world key, tolerance, 5P seconds, LttM seconds
+    ("sph_sph", "minkowski_t,r,θ,ϕ_sphere", 1e-4, 3.048285446, 46.675094802),
+    ("bh_simple", "schwarzschild_t,r,θ,ϕ_simpledisk", 1e-4, 3.360003761, 47.78255016),
+    ("kerr", "kerr_boyer_lindquist", 1e-5, 4.244008561, 68.373113457),
+    ("m87", "m87", 1e-7, 8.032853949, 95.00566409),
+    ("wormhole", "wormhole", 1e-4, 23.663417359, None),
+]
 
 
 def parse():
@@ -211,6 +225,46 @@ def cpu_baseline(w, sample_n, n_poses=1):
                       (len(subset), where, stride, nthreads)}
 
 
+def reference_suite(a):
+    """gen_image of each reference test through the library (host outputs, sRGB
+    conversion included), FP64 and mixed precision, best of `steps` timed runs after
+    `warmup`; one JSON line."""
+    import time
+    from rtwca_amd import abi, host, scenes, skybox
+    ctx = host.Context(0)
+    s1, s2 = skybox.default_skyboxes(1024)  # the reference's 4096-wide skies, synthetic
+    ctx.load_skybox(1, s1)
+    ctx.load_skybox(2, s2)
+    lam = [float(x) for x in range(350, 751, 30)]
+    rows = []
+    for name, key, tol, t5p, tcpu in REFERENCE_SUITE:
+        sc, pos, d, p, beta = scenes.initialize_world(key)
+        cam = abi.make_camera(pos, d, beta, p, a.x_pix or 500, 85, 60)
+        row = {"test": name, "rays": None, "reference_5P_s": t5p, "reference_LttM_s": tcpu}
+        for tag, prec in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
+            it = abi.make_integrator(tol, tol, 0.1, 1e4, prec, abi.SCHED_WAVEFRONT, 3.0)
+            best = None
+            for k in range(a.warmup + a.steps):
+                t0 = time.perf_counter()
+                r = ctx.render(sc, [cam], it, lam, want_class=False)
+                ctx.xyz_to_srgb16(r["xyz"], r["x_pix"], r["y_pix"])
+                dt = time.perf_counter() - t0
+                if k >= a.warmup:
+                    best = dt if best is None else min(best, dt)
+            row["rays"] = r["x_pix"] * r["y_pix"]
+            row[tag + "_s"] = best
+            row[tag + "_kernel_ms"] = r["stats"]["kernel_ms"]
+            row[tag + "_steps"] = int(r["steps"][:, 0].sum())
+            print("  %s %s %.3f s" % (name, tag, best), file=sys.stderr, flush=True)
+        row["speedup_mixed_vs_reference_5P"] = t5p / row["mixed_s"]
+        if tcpu:
+            row["speedup_fp64_vs_reference_LttM"] = tcpu / row["fp64_s"]
+        rows.append(row)
+    print(json.dumps({"reference_suite": rows, "note": "reference times are TestResults.csv on unstated hardware; "
+                      "the wormhole test ran there with ray bundles and the real sky textures, here without "
+                      "bundles and with synthetic 1024-texel skyboxes"}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -225,6 +279,9 @@ def main():
         return
     if a.adjoint_check:
         adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first)
+        return
+    if a.workload == "reference-suite":
+        reference_suite(a)
         return
     idx = WORKLOADS[a.workload]
     x_pix = a.x_pix or (1024 if idx == 3 else 512)

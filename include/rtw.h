@@ -50,8 +50,17 @@ enum rtw_scene_kind {
   RTW_SCENE_KERR_DISK = 3,          /* main/Kerr.source (spin a; Powehi geometry at a = 0.9) */
   RTW_SCENE_KERR_THICK_FF = 4,      /* Kerr + Gaussian thick disk, free-free with van Hoof Gaunt */
   RTW_SCENE_KERR_SYNCH = 5,         /* Kerr + thermal-synchrotron torus and conical jet */
-  RTW_SCENE_POWEHI_M87 = 6          /* main/Powehi.source: the M87* model */
+  RTW_SCENE_POWEHI_M87 = 6,         /* main/Powehi.source: the M87* model */
+  RTW_SCENE_WORMHOLE = 7            /* main/Wormhole.source: Interstellar's Dneg wormhole */
 };
+
+/* RTW_SCENE_WORMHOLE follows main/Wormhole.source: coordinates (t, l, theta, phi) with
+ * l the proper radial distance (negative on the far side), throat radius 1, half-length
+ * 0.005, lensing width 0.05 (source_geometry.jl:7-13), including the reference's
+ * dr_dl (:33-44); no matter.  Rays that end at r(l) > 50 take the texel of skybox 1
+ * (l > 0) or skybox 2 (l < 0) in their direction (skybox_handling! without ray
+ * bundles, source_model.jl:393-443); both must be loaded with rtw_load_skybox.
+ * Only r_escape is read from rtw_scene. */
 
 /* RTW_SCENE_POWEHI_M87 follows main/Powehi.source: Kerr geometry at spin 0.9f0
  * (set RTW_SCENE_REF_F32 for the reference's Float32 evaluation); a wedge disk
@@ -202,6 +211,13 @@ const char* rtw_last_error(const rtw_ctx* ctx);
  * free-free scene interpolates.  Replaces the CSV load at
  * main/emission_functions.jl:15. */
 int rtw_load_gaunt(rtw_ctx* ctx, const char* path);
+
+/* Load a skybox of the wormhole scene: the decoded image as load_textures gives it
+ * (main/Wormhole.source/source_model.jl:10-20, "NASA_Sky_4096.png" as skybox 1 and
+ * "Red_Nebula.png" as skybox 2), 8-bit RGB, row-major, height 3N and width 4N (a
+ * horizontal-cross cube map, face size N = width / 4).  which = 1 or 2.  The texels
+ * stay in device memory for the context's lifetime. */
+int rtw_load_skybox(rtw_ctx* ctx, int32_t which, const uint8_t* rgb, int32_t width, int32_t height);
 
 /* Render n_cams camera poses of one scene.
  * Replaces the integration part of gen_image (main/Iterators/5P/5P.jl:145-215 and

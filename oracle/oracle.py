@@ -59,6 +59,11 @@ def lib():
         L.rtwo_synch_point.argtypes = [P, P, P]
         L.rtwo_radiative.argtypes = [P, P, D, P]
         L.rtwo_fluid.argtypes = [P, P, P, P]
+        L.rtwo_load_skybox.argtypes = [C.c_int, P, C.c_int, C.c_int]
+        L.rtwo_skybox_coord.argtypes = [P, C.c_int, P]
+        L.rtwo_skybox_I.argtypes = [C.c_int, C.c_int, C.c_int, D]
+        L.rtwo_skybox_I.restype = D
+        L.rtwo_wormhole_r.argtypes = [D, P]
         L.rtwo_powehi_consts.argtypes = [P, P]
         rc = L.rtwo_load_gaunt(GAUNT_PATH.encode())
         if rc != 0:
@@ -117,6 +122,33 @@ def radiative(scene, x, nu):
     x8[:len(x)] = x
     out = np.zeros(2)
     lib().rtwo_radiative(_r(scene), _p(x8), float(nu), _p(out))
+    return out[0], out[1]
+
+
+def load_skybox(which, rgb):
+    """rgb: (3N, 4N, 3) uint8 horizontal-cross cube map."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    rc = lib().rtwo_load_skybox(int(which), _p(rgb), rgb.shape[1], rgb.shape[0])
+    if rc != 0:
+        raise ValueError("bad skybox shape %r" % (rgb.shape,))
+
+
+def skybox_coord(ray8, N):
+    """(i, j, u, v, face, i_abs, j_abs) of calc_skybox_coord + cubemap_coord_calc."""
+    out = np.zeros(7)
+    lib().rtwo_skybox_coord(_p(np.ascontiguousarray(ray8, dtype=np.float64)), int(N), _p(out))
+    i, j, u, v, face, ia, ja = out
+    return int(i), int(j), u, v, int(face), int(ia), int(ja)
+
+
+def skybox_I(which, ia, ja, f):
+    return lib().rtwo_skybox_I(int(which), int(ia), int(ja), float(f))
+
+
+def wormhole_r(l):
+    """(r(l), the reference's dr_dl(l))."""
+    out = np.zeros(2)
+    lib().rtwo_wormhole_r(float(l), _p(out))
     return out[0], out[1]
 
 

@@ -77,6 +77,7 @@ inline Cnt sin(Cnt a) { return tr(std::sin(a.v)); }
 inline Cnt cos(Cnt a) { return tr(std::cos(a.v)); }
 inline Cnt tan(Cnt a) { return tr(std::tan(a.v)); }
 inline Cnt pow(Cnt a, Cnt b) { return tr(std::pow(a.v, b.v)); }
+inline Cnt atan(Cnt a) { return tr(std::atan(a.v)); }
 inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
 inline Cnt fmod(Cnt a, Cnt b) { return ar(std::fmod(a.v, b.v)); }
 inline bool isfinite(Cnt a) { return std::isfinite(a.v); }
@@ -84,7 +85,7 @@ inline double val(Cnt a) { return a.v; }
 inline double val(double a) { return a; }
 
 using std::sqrt; using std::exp; using std::log; using std::log10; using std::sin;
-using std::cos; using std::tan; using std::pow; using std::fabs; using std::fmod;
+using std::cos; using std::tan; using std::pow; using std::fabs; using std::fmod; using std::atan;
 using std::isfinite;
 
 template <class R> inline R rmin(R a, R b) { return (b < a) ? b : a; }   // Julia min (no NaN case needed)
@@ -137,6 +138,7 @@ enum SceneKind {
   KERR_THICK_FF = 4,      // new: Kerr geometry + Gaussian thick disk, free-free with van Hoof Gaunt
   KERR_SYNCH = 5,         // new: Kerr geometry + thermal-synchrotron torus and conical jet
   POWEHI_M87 = 6,         // Powehi.source: the M87* model (wedge disk + power-law synchrotron jet)
+  WORMHOLE = 7,           // Wormhole.source: Interstellar's Dneg wormhole, skyboxes on both sides
 };
 enum SceneFlags {
   REF_F32 = 1,  // Kerr family: Float32 Christoffel, horizon/ISCO and max_dt_scale as the reference
@@ -193,12 +195,42 @@ inline double kerr_circ_beta(double r, double a, double M) {
   return std::sqrt(M) * (r * r - 2 * a * std::sqrt(M * r) + a * a) / (std::sqrt(D) * (std::pow(r, 1.5) + a * std::sqrt(M)));
 }
 
+// ---------------------------------------------------------------------------
+// Wormhole.source (Dneg wormhole of Interstellar, source_geometry.jl:7-45): proper
+// radial coordinate l, r(l) = rho + M (x atan x - log(1 + x^2)/2) for |l| > a,
+// x = 2(|l| - a)/(pi M); rho = 1f0, a = 0.005 rho, W = 0.05 rho, M = W / W_over_M.
+// dr_dl is the reference's, 2 sign(l)(atan x + (x - 1)/(1 + x^2))/pi: the derivative
+// of its r(l) is 2 sign(l) atan(x)/pi, so the Christoffel symbols carry an extra term
+// (quirk; the geodesic equation is not that of the metric).
+inline double wh_W_over_M() {
+  const double q = PI / (2 * std::sqrt(2.0));
+  return -std::log(1 / std::cos(q)) + q * std::tan(q);
+}
+constexpr double WH_RHO = 1.0;           // 1.0f0
+constexpr double WH_A = 0.005 * WH_RHO;  // a_wh
+constexpr double WH_W = 0.05 * WH_RHO;
+inline double wh_M() { return WH_W / wh_W_over_M(); }
+template <class R> R wh_x(R l) { return R(2) * (fabs(l) - R(WH_A)) / (R(PI) * R(wh_M())); }
+template <class R> R wh_r(R l) {
+  if (fabs(l) > R(WH_A)) {
+    const R x = wh_x(l);
+    return R(WH_RHO) + R(wh_M()) * (x * atan(x) - log(R(1) + sq(x)) / R(2));
+  }
+  return R(WH_RHO);
+}
+template <class R> R wh_drdl(R l) {
+  if (fabs(l) <= R(WH_A)) return R(0);
+  const R x = wh_x(l);
+  const R sg = R(val(l) > 0 ? 1.0 : (val(l) < 0 ? -1.0 : 0.0));
+  return R(2) * sg * (atan(x) + (x - R(1)) / (R(1) + sq(x))) / R(PI);
+}
+
 inline void synch_init(Scene& s);
 inline void scene_init(Scene& s) {
   s.r_s = 2 * s.mass;
   if (s.kind == SCHWARZSCHILD_DISK) {
     s.r_ev = s.r_s;  // Schwarzschild_...source_model.jl:358 terminates at (1+10 rtol) r_s
-  } else if (s.kind >= KERR_DISK) {
+  } else if (s.kind >= KERR_DISK && s.kind != WORMHOLE) {
     if (s.flags & REF_F32) {  // Kerr.source/source_geometry.jl:7-17 in Float32
       float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
       s.r_ev = (double)(rs / 2 + std::sqrt((rs * rs / 4) - a * a));
@@ -229,6 +261,11 @@ template <class R> void lower_metric(const Scene& s, const R* x, R g[4][4]) {
     g[1][1] = R(1.0) / (R(1.0) - rs / x[1]);
     g[2][2] = sq(x[1]);
     g[3][3] = sq(x[1] * sin(x[2]));
+  } else if (s.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:49-55
+    g[0][0] = R(-1.0);
+    g[1][1] = R(1.0);
+    g[2][2] = sq(wh_r(x[1]));
+    g[3][3] = sq(wh_r(x[1]) * sin(x[2]));
   } else {
     R a(s.spin), rs(s.r_s);
     R Sig = sq(x[1]) + sq(a * cos(x[2]));
@@ -321,6 +358,17 @@ template <class R> void christoffel(const Scene& s, const R* x, R G[4][4][4]) {
     G[3][3][1] = G[3][1][3];
     G[3][2][3] = R(1) / tan(x[2]);
     G[3][3][2] = G[3][2][3];
+  } else if (s.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:129-146
+    const R rl = wh_r(x[1]), drdl = wh_drdl(x[1]), th = x[2];
+    G[1][2][2] = -rl * drdl;
+    G[1][3][3] = -rl * drdl * sq(sin(th));
+    G[2][1][2] = (R(1) / rl) * drdl;
+    G[2][2][1] = G[2][1][2];
+    G[2][3][3] = -sin(th) * cos(th);
+    G[3][1][3] = (R(1) / rl) * drdl;
+    G[3][3][1] = G[3][1][3];
+    G[3][2][3] = R(1) / tan(th);
+    G[3][3][2] = G[3][2][3];
   } else if (s.flags & REF_F32) {
     // Q3: r and theta cast to Float32 and evaluated in Float32 (Kerr.source:196-197)
     float Gf[4][4][4];
@@ -351,8 +399,9 @@ template <class R> std::pair<bool, R> near_singularity(const Scene& s, const R* 
       return {true, R(2) * R(atol2) / (fabs(ray[5] * R(37)) + d)};
     return {false, h};
   }
-  if (fabs(th_new) <= R(atol3)) return {true, R(2) * fabs(ray[2]) / (fabs(ray[6] * R(37)) + d)};
-  if (fabs(th_new - pi) <= R(atol3)) return {true, R(2) * fabs((ray[2] - pi) / (fabs(ray[6] * R(37)) + d))};
+  const R lim = (s.kind == WORMHOLE) ? R(5) * R(atol3) : R(atol3);  // Wormhole: 5 abs_tol (:158-171)
+  if (fabs(th_new) <= lim) return {true, R(2) * fabs(ray[2]) / (fabs(ray[6] * R(37)) + d)};
+  if (fabs(th_new - pi) <= lim) return {true, R(2) * fabs((ray[2] - pi) / (fabs(ray[6] * R(37)) + d))};
   return {false, h};
 }
 
@@ -392,7 +441,7 @@ template <class R> R pad_max_dt(const Scene& s, const R* ray, double max_dt_scal
   const double tenth_f = (double)0.1f, half_f = 0.5, tiny_f = (double)1e-12f;
   R d(NO_DIV_ZERO), pi(PI);
   R eps;
-  bool kerr = s.kind >= KERR_DISK;
+  bool kerr = s.kind >= KERR_DISK && s.kind != WORMHOLE;
   double mds = (kerr && (s.flags & REF_F32)) ? (double)(float)max_dt_scale : max_dt_scale;
   if (ray[2] < R(PI / 8) || ray[2] > R(7 * PI / 8)) {
     eps = R(std::min(mds, tenth_f));
@@ -401,6 +450,7 @@ template <class R> R pad_max_dt(const Scene& s, const R* ray, double max_dt_scal
   } else {
     eps = R(mds);
   }
+  if (s.kind == WORMHOLE && fabs(ray[1]) < R(100 * WH_A)) eps = eps / R(10);  // Wormhole source_model.jl:100-102
   R A = kerr ? fabs(R(1) / (eps / fabs(ray[5])) + d) : fabs(R(1) / (eps / (fabs(ray[5]) + d)) + d);
   R B = fabs(R(1) / (eps * rmin(ray[2], pi - ray[2]) / (fabs(ray[6]) + d) + d));
   R C = fabs(R(1) / (eps / (fabs(ray[7]) + d) + d));
@@ -641,7 +691,7 @@ template <class R> bool synch_velocity(const Scene& s, const R* x, int reg, R u[
 // get_source_velocity!: Minkowski source_model.jl:288-294, Schwarzschild :312-333,
 // Kerr :109-137 (prograde Keplerian in matter, FIDO outside).
 template <class R> void source_velocity(const Scene& s, const R* x, const R g[4][4], R u[4]) {
-  if (s.kind == MINKOWSKI_SPHERE) {
+  if (s.kind == MINKOWSKI_SPHERE || s.kind == WORMHOLE) {  // static observer (Wormhole source_model.jl:111-118)
     u[0] = R(1); u[1] = R(0); u[2] = R(0); u[3] = R(0);
     return;
   }
@@ -922,8 +972,10 @@ template <class R> struct RaySys {
   void keep(R* ray) const { keepinbounds(*sc, ray); }
   // calc_terminate: Minkowski source_model.jl:308-322, Schwarzschild :356-370, Kerr :168-182.
   bool terminate(const R* ray, double max_steps, double step_count) const {
-    bool horizon = (sc->kind != MINKOWSKI_SPHERE) && (ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev));
-    if (step_count >= max_steps || ray[1] > R(sc->r_escape) || horizon || ray[3] > R(8 * PI)) return true;
+    bool horizon = (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE) && (ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev));
+    // the wormhole tests r(l) (Wormhole.source/source_model.jl:136)
+    const R rr = (sc->kind == WORMHOLE) ? wh_r(ray[1]) : ray[1];
+    if (step_count >= max_steps || rr > R(sc->r_escape) || horizon || ray[3] > R(8 * PI)) return true;
     R thr(-std::log(o.atol));
     for (int i = 9; i < n; i += 2)
       if (ray[i] < thr) return false;
@@ -931,8 +983,10 @@ template <class R> struct RaySys {
   }
   int classify(const R* ray) const {
     if (!isfinite(ray[1])) return CLS_BOUNDARY;
-    if (sc->kind != MINKOWSKI_SPHERE && ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev)) return CLS_HORIZON;
-    if (ray[1] > R(sc->r_escape)) return CLS_ESCAPE;
+    if (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE && ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev))
+      return CLS_HORIZON;
+    if (sc->kind != WORMHOLE && ray[1] > R(sc->r_escape)) return CLS_ESCAPE;
+    if (sc->kind == WORMHOLE && wh_r(ray[1]) > R(sc->r_escape)) return CLS_ESCAPE;
     R thr(-std::log(o.atol));
     bool thick = true;
     for (int i = 9; i < n; i += 2)
@@ -1046,6 +1100,88 @@ template <class R> RayStats integrate_ray(RaySys<R>& sys, R* ray) {
   }
   st.cls = sys.classify(ray);
   return st;
+}
+
+// ---------------------------------------------------------------------------
+// Skyboxes (Wormhole.source/source_model.jl:10-20, 150-276, 393-443; gen_geometry.jl:
+// 110-123, 347-370).  A skybox is a horizontal-cross cube map of 3N rows x 4N columns of
+// 8-bit RGB (the reference loads PNGs with N = width / 4); skybox 1 is seen on the
+// l > 0 side, skybox 2 on the l < 0 side.
+struct Skybox {
+  int N = 0;
+  std::vector<uint8_t> rgb;  // row-major [3N][4N][3]
+};
+inline Skybox& skybox(int which) {
+  static Skybox s[2];
+  return s[which - 1];
+}
+
+// Normal_Gauss (main/my_special_functions.jl:20-22)
+inline double normal_gauss(double x, double sg, double mu) {
+  return std::exp(-sq(x - mu) / (2 * sq(sg))) / (sg * std::sqrt(2 * PI));
+}
+
+// skybox_I_nu_calc (:150-172): RGB of the texel at absolute cube-map (i, j) (1-based;
+// row 3N + 1 - j, column i) spread into Gaussian lines at 630, 532 and 467 nm.
+inline double skybox_I_nu(const Skybox& sb, int i, int j, double f) {
+  const size_t row = (size_t)(3 * sb.N - j), col = (size_t)(i - 1);
+  const uint8_t* px = &sb.rgb[(row * 4 * sb.N + col) * 3];
+  const double R = 1.0 * (px[0] / 255.0), G = 0.843 * (px[1] / 255.0), B = 0.914 * (px[2] / 255.0);
+  return 1e-12 * (R * normal_gauss(f, 5e-5, 0.001587) + G * normal_gauss(f, 7e-5, 0.001879) +
+                  B * normal_gauss(f, 9e-5, 0.002141));
+}
+
+// calc_skybox_coord (:175-276): cube face and in-face pixel (1-based) of the direction
+// R_sph_to_cart e k (vierbein diag(1, 1, r(l), r(l) sin theta)).
+inline void skybox_coord(const double* ray, int N, int& i, int& j, double& u, double& v, int& face) {
+  const double rl = wh_r(ray[1]);
+  const double st = std::sin(ray[2]), ct = std::cos(ray[2]), sp = std::sin(ray[3]), cp = std::cos(ray[3]);
+  const double e2 = rl, e3 = rl * st;  // vierbein diagonal (t, l entries are 1)
+  // (R e) k with R = R_sph_to_cart (gen_geometry.jl:110-123)
+  const double vx = (st * cp) * ray[5] + (ct * cp * e2) * ray[6] + (-sp * e3) * ray[7];
+  const double vy = (st * sp) * ray[5] + (ct * sp * e2) * ray[6] + (cp * e3) * ray[7];
+  const double vz = ct * ray[5] + (-st * e2) * ray[6];
+  const double nrm = std::sqrt(vx * vx + vy * vy + vz * vz);
+  const double x = vx / nrm, y = vy / nrm, z = vz / nrm;
+  const double ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
+  if (ax >= ay && ax >= az) face = x > 0 ? 1 : 3;
+  else if (ay >= ax && ay >= az) face = y > 0 ? 4 : 2;
+  else face = z > 0 ? 6 : 5;
+  switch (face) {
+    case 1: u = -y / ax; v = z / ax; break;
+    case 2: u = -x / ay; v = z / ay; break;
+    case 3: u = y / ax; v = z / ax; break;
+    case 4: u = x / ay; v = z / ay; break;
+    case 5: u = -x / az; v = -y / az; break;
+    default: u = -x / az; v = y / az; break;
+  }
+  i = std::min((int)std::floor(N * (u + 1) / 2) + 1, N);
+  j = std::min((int)std::floor(N * (v + 1) / 2) + 1, N);
+}
+
+// cubemap_coord_calc (gen_geometry.jl:347-370): face-local -> absolute cross coordinates
+inline void cubemap_abs(int i, int j, int face, int N, int& ia, int& ja) {
+  static const int di[7] = {0, 0, 1, 2, 3, 1, 1}, dj[7] = {0, 1, 1, 1, 1, 0, 2};
+  ia = i + di[face] * N;
+  ja = j + dj[face] * N;
+}
+
+// skybox_handling! without ray bundles (:393-443): rays that ended at r(l) > 50 take the
+// intensity of their skybox texel, attenuated by exp(-tau), per colour.
+inline void skybox_handling(const Scene& s, double* ray, int ncol, const double* freq) {
+  if (s.kind != WORMHOLE) return;
+  const double rl = wh_r(ray[1]);
+  if (!(rl > 50) || ray[1] == 0) return;
+  const Skybox& sb = skybox(ray[1] > 0 ? 1 : 2);
+  const double fs = ray[4];  // -[1,0,0,0]^T g k with g_tt = -1
+  int i, j, face, ia, ja;
+  double u, v;
+  skybox_coord(ray, sb.N, i, j, u, v, face);
+  cubemap_abs(i, j, face, sb.N, ia, ja);
+  for (int c = 0; c < ncol; ++c) {
+    const double f = freq[c] * fs;
+    ray[8 + 2 * c] += skybox_I_nu(sb, ia, ja, f) * std::exp(-ray[9 + 2 * c]) / cube(f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1191,7 +1327,10 @@ inline void boost_z(double beta, M4 L) {
 // calc_inv_vierbein: Minkowski source_geometry.jl:45-50, Schwarzschild :53-60, Kerr :112-121.
 inline void inv_vierbein(const Scene& s, const double* x, M4 e) {
   m4_eye(e);
-  if (s.kind == MINKOWSKI_SPHERE) {
+  if (s.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:83-90
+    e[2][2] = 1 / wh_r(x[1]);
+    e[3][3] = 1 / (wh_r(x[1]) * std::sin(x[2]) + NO_DIV_ZERO);
+  } else if (s.kind == MINKOWSKI_SPHERE) {
     e[2][2] = 1 / (x[1] + NO_DIV_ZERO);
     e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
   } else if (s.kind == SCHWARZSCHILD_DISK) {

@@ -43,6 +43,16 @@ static Opts to_opts(const rtw_integrator* p) {
 
 extern "C" {
 
+// A skybox as the reference's load_textures gives it: 8-bit RGB, height 3N rows,
+// width 4N columns, row-major.
+int rtwo_load_skybox(int which, const uint8_t* rgb, int width, int height) {
+  if (which < 1 || which > 2 || width % 4 != 0 || height * 4 != width * 3 || !rgb) return -1;
+  Skybox& sb = skybox(which);
+  sb.N = width / 4;
+  sb.rgb.assign(rgb, rgb + (size_t)width * height * 3);
+  return 0;
+}
+
 // Parse the van Hoof table the way emission_functions.jl:15 does: skip 42 lines, then
 // skip comment lines, then 146 rows of 81 numbers, stored as Float32.
 int rtwo_load_gaunt(const char* path) {
@@ -155,6 +165,19 @@ void rtwo_fluid(const rtw_scene* s, const double* ray8, double* out8, double* u4
   fluid_params<double>(sc, ray8, u4, g, fs, fp);
   for (int i = 0; i < 8; ++i) out8[i] = fp[i];
 }
+// Skybox lookups: out7 = (i, j, u, v, face, i_abs, j_abs) of a ray state for face size N;
+// skybox_I_nu of skybox `which` at absolute (i, j) and frequency f.
+void rtwo_skybox_coord(const double* ray8, int N, double* out7) {
+  int i, j, face, ia, ja;
+  double u, v;
+  skybox_coord(ray8, N, i, j, u, v, face);
+  cubemap_abs(i, j, face, N, ia, ja);
+  out7[0] = i; out7[1] = j; out7[2] = u; out7[3] = v; out7[4] = face; out7[5] = ia; out7[6] = ja;
+}
+double rtwo_skybox_I(int which, int ia, int ja, double f) { return skybox_I_nu(skybox(which), ia, ja, f); }
+// Wormhole r(l) and the reference's dr_dl.
+void rtwo_wormhole_r(double l, double* out2) { out2[0] = wh_r<double>(l); out2[1] = wh_drdl<double>(l); }
+
 // Powehi constants: Approx_P_synch(2.5), Omega_jet, nozzle acceleration.
 void rtwo_powehi_consts(const rtw_scene* s, double* out3) {
   Scene sc = to_scene(s);
@@ -276,6 +299,7 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
                 int nthreads, const int64_t* subset, int64_t nsub, const double* init_rays, double* out_xyz,
                 uint8_t* out_cls, int64_t* out_steps, double* out_rays, double* out_seconds) {
   if (!gaunt_table().loaded && s->kind == KERR_THICK_FF) return -10;
+  if (s->kind == WORMHOLE && (skybox(1).N == 0 || skybox(2).N == 0)) return -11;
   Scene sc = to_scene(s);
   Camera cam = to_cam(c);
   Opts op = to_opts(o);
@@ -309,6 +333,7 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
         }
         RaySys<double> sys(&sc, op, nlam, freq.data());
         RayStats st = integrate_ray(sys, ray.data());
+        skybox_handling(sc, ray.data(), nlam, freq.data());  // LttM.jl:54
         double xyz[3];
         calc_xyz(ray.data(), lam, freq.data(), nlam, WX.data(), WY.data(), WZ.data(), xyz);
         if (out_xyz) for (int k = 0; k < 3; ++k) out_xyz[3 * p + k] = xyz[k];

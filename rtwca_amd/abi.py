@@ -20,6 +20,7 @@ SCENE_KERR_DISK = 3
 SCENE_KERR_THICK_FF = 4
 SCENE_KERR_SYNCH = 5
 SCENE_POWEHI_M87 = 6
+SCENE_WORMHOLE = 7
 
 SCENE_REF_F32 = 1
 SCENE_GAUNT_FP64 = 2

@@ -53,6 +53,8 @@ struct rtw_ctx {
   int num_cu = 0;
   std::string err;
   bool gaunt_loaded = false;
+  DevBuf sky[2];        // wormhole skyboxes (3N x 4N x 3 bytes each)
+  int sky_n[2] = {0, 0};
   std::vector<float> gaunt;  // 146 x 81 row-major
   DevBuf cams, colors, gcols, queue, counters, xyz, cls, steps, rays, init, aux, kargs, pool;
   DevBuf adj_work, adj_info, adj_tinfo, adj_offs, adj_tape, adj_ckpt, adj_seg, adj_lam, adj_dl, adj_grad;
@@ -159,6 +161,15 @@ static void inv_vierbein(const DevScene& S, const double* x, M4 e) {
   if (S.kind == MINK) {
     e[2][2] = 1 / (x[1] + NO_DIV_ZERO);
     e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
+  } else if (S.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:83-90
+    const double l = x[1], a = S.wh_a;
+    double rl = S.wh_rho;
+    if (std::fabs(l) > a) {
+      const double q = 2 * (std::fabs(l) - a) / (M_PI * S.wh_M);
+      rl = S.wh_rho + S.wh_M * (q * std::atan(q) - std::log(1 + q * q) / 2);
+    }
+    e[2][2] = 1 / rl;
+    e[3][3] = 1 / (rl * std::sin(x[2]) + NO_DIV_ZERO);
   } else if (S.kind == SCHW) {
     double f = 1.0 - S.rs / x[1];
     e[0][0] = 1 / std::sqrt(f);
@@ -177,8 +188,12 @@ static void inv_vierbein(const DevScene& S, const double* x, M4 e) {
   }
 }
 
-static bool cam_is_singular(const double* x) {
-  return std::fabs(x[1]) <= (double)1e-32f || std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - M_PI) <= 1e-32;
+// is_singularity (Kerr source_geometry.jl:307-315 and the others; the wormhole's
+// Wormhole.source/source_geometry.jl:198-205 tests theta only, l = 0 is the throat)
+static bool cam_is_singular(const double* x, int kind = 0) {
+  const bool pole = std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - M_PI) <= 1e-32;
+  if (kind == RTW_SCENE_WORMHOLE) return pole;
+  return std::fabs(x[1]) <= (double)1e-32f || pole;
 }
 
 static int cam_y_pix(const rtw_camera& c) {
@@ -312,10 +327,20 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.r_escape = s.r_escape;
   d.r_photon15 = 1.5 * (3 * s.mass);
   d.ref_f32 = (s.kind >= RTW_SCENE_KERR_DISK && (s.flags & RTW_SCENE_REF_F32)) ? 1 : 0;
-  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_POWEHI_M87)
+  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_WORMHOLE)
     return fail(ctx, RTW_E_UNSUPPORTED, "scene kind not supported");
   if (s.kind == RTW_SCENE_SCHWARZSCHILD_DISK) {
     d.r_ev = d.rs;
+  } else if (s.kind == RTW_SCENE_WORMHOLE) {
+    // Wormhole.source/source_geometry.jl:7-13: rho = 1f0, a = 0.005 rho, W = 0.05 rho,
+    // M = W / W_over_M with W_over_M = -log(sec q) + q tan q, q = pi / (2 sqrt 2)
+    const double q = M_PI / (2 * std::sqrt(2.0));
+    d.wh_rho = 1.0;
+    d.wh_a = 0.005 * d.wh_rho;
+    d.wh_M = (0.05 * d.wh_rho) / (-std::log(1 / std::cos(q)) + q * std::tan(q));
+    d.r_ev = 0;
+    if (!(ctx->sky_n[0] > 0 && ctx->sky_n[1] > 0))
+      return fail(ctx, RTW_E_ARG, "wormhole scene needs both skyboxes (rtw_load_skybox)");
   } else if (s.kind >= RTW_SCENE_KERR_DISK) {
     if (d.ref_f32) {
       float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
@@ -328,6 +353,7 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
     if (!(std::fabs(d.a) < s.mass)) return fail(ctx, RTW_E_ARG, "|spin| must be < M");
   }
   d.r_horizon = (s.kind == RTW_SCENE_MINKOWSKI_SPHERE) ? 0.0 : (1 + 10 * o.rtol) * d.r_ev;
+  if (s.kind == RTW_SCENE_WORMHOLE) d.r_horizon = -HUGE_VAL;  // no horizon; l < 0 is the far side
   // pad_max_dt epsilons: Kerr uses Float32(max_dt_scale) (Kerr.source/source_model.jl:93-95)
   double mds = (d.ref_f32) ? (double)(float)o.max_dt_scale : o.max_dt_scale;
   const double tenth_f = (double)0.1f;
@@ -571,6 +597,21 @@ int rtw_load_gaunt(rtw_ctx* c, const char* path) {
   return RTW_OK;
 }
 
+// A skybox as main/Wormhole.source/source_model.jl:10-20 loads it: the decoded image,
+// 8-bit RGB, height 3N x width 4N, row-major (horizontal cross).
+int rtw_load_skybox(rtw_ctx* c, int32_t which, const uint8_t* rgb, int32_t width, int32_t height) {
+  if (!c || !rgb) return fail(c, RTW_E_ARG, "NULL argument");
+  if (which != 1 && which != 2) return fail(c, RTW_E_ARG, "skybox index must be 1 or 2");
+  if (width < 4 || width % 4 != 0 || (long long)height * 4 != (long long)width * 3)
+    return fail(c, RTW_E_ARG, "skybox must be 4N wide and 3N high");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t bytes = (size_t)width * height * 3;
+  HIPCHK(c, c->sky[which - 1].ensure(bytes));
+  HIPCHK(c, hipMemcpy(c->sky[which - 1].p, rgb, bytes, hipMemcpyHostToDevice));
+  c->sky_n[which - 1] = width / 4;
+  return RTW_OK;
+}
+
 // Common set-up + launch.  Outputs are device pointers.
 static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32_t n_cams,
                           const rtw_integrator* opts, const double* lam, int32_t n_lambda, const double* init_rays,
@@ -593,7 +634,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   for (int k = 0; k < n_cams; ++k) {
     if (cams[k].x_pix != c0.x_pix || cams[k].fov_hor != c0.fov_hor || cams[k].fov_vert != c0.fov_vert)
       return fail(c, RTW_E_ARG, "all poses must share x_pix and field of view");
-    if (cam_is_singular(cams[k].pos)) return fail(c, RTW_E_CAMERA, "camera on a coordinate singularity");
+    if (cam_is_singular(cams[k].pos, scene->kind)) return fail(c, RTW_E_CAMERA, "camera on a coordinate singularity");
     if (!(std::fabs(cams[k].beta) < 1)) return fail(c, RTW_E_ARG, "|beta| must be < 1");
   }
   HIPCHK(c, hipSetDevice(c->device));
@@ -648,6 +689,10 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   A.out_steps = d_steps;
   A.out_rays = d_rays;
   A.counters = c->counters.as<unsigned long long>();
+  for (int b = 0; b < 2; ++b) {
+    A.sky.rgb[b] = c->sky[b].as<uint8_t>();
+    A.sky.N[b] = c->sky_n[b];
+  }
   const int persistent = opts->schedule != RTW_SCHED_GRID ? 1 : 0;
   A.wavefront = opts->schedule == RTW_SCHED_WAVEFRONT ? 1 : 0;
   const bool mixed = opts->precision == RTW_PREC_MIXED;

@@ -27,7 +27,7 @@ constexpr double GAM2_FACTOR = 2.1798723611035e-18 / K_B;         // Ry / k_B
 constexpr double FF_T0 = 1e4;           // thick disk opacity normalisation temperature
 constexpr double FF_NU0 = 1.0 / 550.0;  // and frequency (nm^-1)
 
-enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5, POWEHI = 6 };
+enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5, POWEHI = 6, WORMHOLE = 7 };
 constexpr int SCENE_GAUNT_FP64 = 2;  // Gaunt interpolation in FP64 (scene flag)
 enum { CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
 
@@ -59,6 +59,9 @@ struct DevScene {
   // j and a (power law p = 2.5 with its Gamma functions and 1 / Approx_P_synch(p));
   // the exponents -(p-1)/2, -(p+2)/2 and -p/2
   double pw_omj, pw_noz, pw_rjet, pw_bphi, pw_cj, pw_ca, pw_ej, pw_ea, pw_ephi;
+  // wormhole (Wormhole.source/source_geometry.jl:7-13): throat radius rho,
+  // half-length a and the M of its lensing width
+  double wh_rho, wh_a, wh_M;
 };
 
 struct DevOpts {
@@ -243,6 +246,28 @@ __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4
 template <int KIND, class T>
 __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T x[8], StageOut<T>& o);
 
+// Wormhole r(l) and the reference's dr_dl (Wormhole.source/source_geometry.jl:16-44):
+// x = 2(|l| - a)/(pi M), r = rho + M (x atan x - log(1 + x^2)/2) beyond the throat and
+// dr/dl = 2 sign(l)(atan x + (x - 1)/(1 + x^2))/pi (the reference's form, which is not
+// the derivative of its r(l): that would be 2 sign(l) atan(x)/pi).
+template <class T> __device__ __forceinline__ T wh_r(const DevScene& S, T l) {
+  if (!(fabs(l) > T(S.wh_a))) return T(S.wh_rho);
+  const T x = T(2) * (fabs(l) - T(S.wh_a)) / (T(PI) * T(S.wh_M));
+  return T(S.wh_rho) + T(S.wh_M) * (x * atan(x) - log(T(1) + x * x) / T(2));
+}
+template <class T> __device__ __forceinline__ T wh_drdl(const DevScene& S, T l) {
+  if (fabs(l) <= T(S.wh_a)) return T(0);
+  const T x = T(2) * (fabs(l) - T(S.wh_a)) / (T(PI) * T(S.wh_M));
+  const T sg = l > T(0) ? T(1) : (l < T(0) ? T(-1) : T(0));
+  return T(2) * sg * (atan(x) + (x - T(1)) / (T(1) + x * x)) / T(PI);
+}
+// The radial coordinate the termination tests compare with r_escape: r(l) for the
+// wormhole (Wormhole.source/source_model.jl:136), r otherwise.
+template <int KIND> __device__ __forceinline__ double radial(const DevScene& S, double r) {
+  if constexpr (KIND == WORMHOLE) return wh_r<double>(S, r);
+  else return r;
+}
+
 // Geodesic right-hand side and stage scalars at x (calc_ray_derivative!,
 // main/Iterators/LttM/integrators_LttM.jl:11-36).
 template <int KIND, class T>
@@ -265,6 +290,20 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     // metric (-1,1,r^2,...), static source: shift = k^t (source_model.jl:288-294)
     o.sh = kt;
     o.matter = r <= T(S.r_out);
+  } else if constexpr (KIND == WORMHOLE) {
+    // Wormhole.source/source_geometry.jl:129-146; static observer (source_model.jl:111-118),
+    // no matter anywhere
+    T s, c;
+    sc_(th, s, c);
+    const T rl = wh_r<T>(S, r), drdl = wh_drdl<T>(S, r);
+    const T rr = rl * drdl, ird = drdl * rcp_<F>(rl);
+    const T cot = c * rcp_<F>(s);
+    o.d[4] = T(0);
+    o.d[5] = rr * kth * kth + rr * (s * s) * kph * kph;
+    o.d[6] = T(-2) * ird * kr * kth + s * c * kph * kph;
+    o.d[7] = T(-2) * (ird * kr * kph + cot * kth * kph);
+    o.sh = kt;
+    o.matter = false;
   } else if constexpr (KIND == SCHW) {
     // Schwarzschild_Simple_Disk_Spherical.source/source_geometry.jl:109-128
     T s, c;
@@ -472,7 +511,9 @@ __device__ __forceinline__ double pad_max_dt(const DevScene& S, const double* x)
   if (x[2] < PI / 8 || x[2] > 7 * PI / 8) eps = S.eps_pole;
   else if (KIND == SCHW && x[1] <= S.r_photon15) eps = S.eps_photon;
   else eps = S.eps_main;
-  double A = (KIND >= KERR) ? fabs(1.0 / (eps / fabs(x[5])) + d) : fabs(1.0 / (eps / (fabs(x[5]) + d)) + d);
+  if (KIND == WORMHOLE && fabs(x[1]) < 100 * S.wh_a) eps = eps / 10;  // Wormhole source_model.jl:100-102
+  double A = (KIND >= KERR && KIND != WORMHOLE) ? fabs(1.0 / (eps / fabs(x[5])) + d)
+                                                : fabs(1.0 / (eps / (fabs(x[5]) + d)) + d);
   double th = x[2];
   double mth = (PI - th < th) ? PI - th : th;
   double B = fabs(1.0 / (eps * mth / (fabs(x[6]) + d) + d));
@@ -499,8 +540,9 @@ __device__ __forceinline__ bool near_singularity(const double* x, double h, doub
     if (fabs(new_r) <= 100 * atol) { ts = 2 * atol / (fabs(x[5] * 37) + d); return true; }
     return false;
   }
-  if (fabs(th_new) <= atol) { ts = 2 * fabs(x[2]) / (fabs(x[6] * 37) + d); return true; }
-  if (fabs(th_new - PI) <= atol) { ts = 2 * fabs((x[2] - PI) / (fabs(x[6] * 37) + d)); return true; }
+  const double lim = (KIND == WORMHOLE) ? 5 * atol : atol;  // Wormhole source_geometry.jl:158-171
+  if (fabs(th_new) <= lim) { ts = 2 * fabs(x[2]) / (fabs(x[6] * 37) + d); return true; }
+  if (fabs(th_new - PI) <= lim) { ts = 2 * fabs((x[2] - PI) / (fabs(x[6] * 37) + d)); return true; }
   return false;
 }
 
@@ -654,6 +696,67 @@ __device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, i
 }
 
 // Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
+// Skybox lookup (Wormhole.source/source_model.jl:150-276, 393-443; gen_geometry.jl:
+// 110-123, 347-370): a ray that ended at r(l) > 50 takes the texel of skybox 1 (l > 0)
+// or 2 (l < 0) in the direction R_sph_to_cart e k.  Returns false when none applies;
+// otherwise the texel's weighted RGB (1, 0.843, 0.914 x value/255) and the shift k^t.
+struct DevSky {
+  const uint8_t* rgb[2];  // [3N][4N][3] each, row-major
+  int N[2];
+};
+__device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K, const double* R, double& fs,
+                                             double w[3]) {
+  const double rl = wh_r<double>(S, R[1]);
+  if (!(rl > 50) || R[1] == 0) return false;
+  const int b = R[1] > 0 ? 0 : 1;
+  const int N = K.N[b];
+  double st, ct, sp, cp;
+  sincos(R[2], &st, &ct);
+  sincos(R[3], &sp, &cp);
+  const double e2 = rl, e3 = rl * st;
+  const double vx = (st * cp) * R[5] + (ct * cp * e2) * R[6] + (-sp * e3) * R[7];
+  const double vy = (st * sp) * R[5] + (ct * sp * e2) * R[6] + (cp * e3) * R[7];
+  const double vz = ct * R[5] + (-st * e2) * R[6];
+  const double nrm = sqrt(vx * vx + vy * vy + vz * vz);
+  const double x = vx / nrm, y = vy / nrm, z = vz / nrm;
+  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  int face;
+  double u, v;
+  if (ax >= ay && ax >= az) face = x > 0 ? 1 : 3;
+  else if (ay >= ax && ay >= az) face = y > 0 ? 4 : 2;
+  else face = z > 0 ? 6 : 5;
+  switch (face) {
+    case 1: u = -y / ax; v = z / ax; break;
+    case 2: u = -x / ay; v = z / ay; break;
+    case 3: u = y / ax; v = z / ax; break;
+    case 4: u = x / ay; v = z / ay; break;
+    case 5: u = -x / az; v = -y / az; break;
+    default: u = -x / az; v = y / az; break;
+  }
+  if (!(fabs(u) <= 1.0) || !(fabs(v) <= 1.0)) return false;  // non-finite direction: no texel
+  int i = (int)floor(N * (u + 1) / 2) + 1, j = (int)floor(N * (v + 1) / 2) + 1;
+  i = i < N ? i : N;
+  j = j < N ? j : N;
+  // cubemap_coord_calc: face offsets in the horizontal cross
+  const int di = (face == 2 || face == 5 || face == 6) ? 1 : (face == 3 ? 2 : (face == 4 ? 3 : 0));
+  const int dj = (face == 5) ? 0 : (face == 6 ? 2 : 1);
+  const long long row = 3LL * N - (j + dj * N), col = (long long)(i + di * N) - 1;
+  const uint8_t* px = K.rgb[b] + (row * 4LL * N + col) * 3;
+  w[0] = 1.0 * (px[0] / 255.0);
+  w[1] = 0.843 * (px[1] / 255.0);
+  w[2] = 0.914 * (px[2] / 255.0);
+  fs = R[4];
+  return true;
+}
+// Normal_Gauss (main/my_special_functions.jl:20-22) and skybox_I_nu_calc's line mix
+__device__ __forceinline__ double normal_gauss(double x, double sg, double mu) {
+  return exp(-((x - mu) * (x - mu)) / (2 * (sg * sg))) / (sg * sqrt(2 * PI));
+}
+__device__ __forceinline__ double skybox_I(const double w[3], double f) {
+  return 1e-12 * (w[0] * normal_gauss(f, 5e-5, 0.001587) + w[1] * normal_gauss(f, 7e-5, 0.001879) +
+                  w[2] * normal_gauss(f, 9e-5, 0.002141));
+}
+
 struct StageSc {
   double sh;   // shift
   double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4; synchrotron: -Q n / shift;

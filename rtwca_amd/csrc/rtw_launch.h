@@ -28,6 +28,7 @@ RTW_DECLARE_KIND(launch_kind_kerr)
 RTW_DECLARE_KIND(launch_kind_kerr_ff)
 RTW_DECLARE_KIND(launch_kind_kerr_synch)
 RTW_DECLARE_KIND(launch_kind_powehi)
+RTW_DECLARE_KIND(launch_kind_wormhole)
 
 template <class K> hipError_t persistent_grid(K kern, size_t lds, long long need, int num_cu, long long& grid, int& bpc) {
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, BLOCK, lds);
@@ -112,6 +113,7 @@ inline hipError_t launch_pass(int which, const KArgs& A, const KArgs* dA, const 
     case KERR_FF: return launch_kind_kerr_ff(which, A, dA, Q, mixed, num_cu, st, li);
     case KERR_SYNCH: return launch_kind_kerr_synch(which, A, dA, Q, mixed, num_cu, st, li);
     case POWEHI: return launch_kind_powehi(which, A, dA, Q, mixed, num_cu, st, li);
+    case WORMHOLE: return launch_kind_wormhole(which, A, dA, Q, mixed, num_cu, st, li);
     default: li.msg = "unsupported scene"; return hipErrorInvalidValue;
   }
 }

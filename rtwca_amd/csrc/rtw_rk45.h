@@ -90,6 +90,7 @@ struct KArgs {
   int32_t* out_steps;
   double* out_rays;         // optional (total, n) column-major
   unsigned long long* counters;  // attempted, accepted, fp64, transfer, lane-group steps
+  DevSky sky;               // wormhole skyboxes
   int wavefront;            // rays hand frozen-vacuum stretches to vacuum_kernel
   RayPool pool;
 };
@@ -351,7 +352,9 @@ struct Integrator {
 
   // calc_terminate (Kerr.source/source_model.jl:168-182 and the other scenes).
   __device__ __forceinline__ bool stop_test(double r, double phi, double max_steps, double step_count) const {
-    if (step_count >= max_steps || r > S.r_escape || (KIND != MINK && r <= S.r_horizon) || phi > 8 * PI) return true;
+    if (step_count >= max_steps || radial<KIND>(S, r) > S.r_escape || (KIND != MINK && r <= S.r_horizon) ||
+        phi > 8 * PI)
+      return true;
     return all_thick();
   }
   __device__ __forceinline__ bool terminate(double max_steps, double step_count) const {
@@ -362,7 +365,7 @@ struct Integrator {
     bool thick = all_thick();
     if (!isfinite(rr)) return CLS_BOUNDARY;
     if (KIND != MINK && rr <= S.r_horizon) return CLS_HORIZON;
-    if (rr > S.r_escape) return CLS_ESCAPE;
+    if (radial<KIND>(S, rr) > S.r_escape) return CLS_ESCAPE;
     if (thick && A.O.ncol > 0) return CLS_OPAQUE;
     return CLS_BOUNDARY;
   }
@@ -761,6 +764,18 @@ struct Integrator {
 
   // calc_xyY sums (color_theory.jl:67-102) and the outputs of one ray.
   __device__ __forceinline__ void finish(long long ray, int attempted, int accepted) {
+    if constexpr (KIND == WORMHOLE) {  // skybox_handling! (LttM.jl:54)
+      double R[8], fs, w[3];
+      gather(Ro, R);
+      if (skybox_texel(S, A.sky, R, fs, w)) {
+#pragma unroll
+        for (int m = 0; m < CMAX; ++m) {
+          if (!col_ok(m)) continue;
+          const double f = scol[lane + m * L].f * fs;
+          sI[m] += skybox_I(w, f) * exp(-sT[m]) / (f * f * f);
+        }
+      }
+    }
     double X = 0, Y = 0, Z = 0;
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) {

@@ -160,7 +160,8 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
     h = nh;
     rej = !accept;
     // calc_terminate without the optical-depth test (frozen tau was not thick)
-    if ((double)att >= max_steps || R[1] > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) || R[3] > 8 * PI) {
+    if ((double)att >= max_steps || radial<KIND>(S, R[1]) > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) ||
+        R[3] > 8 * PI) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) P.geo[ray * 8 + j] = R[j];
       P.h[ray] = h;
@@ -187,9 +188,22 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   if (ray >= N) return;
   double X = 0, Y = 0, Z = 0;
   bool thick = A.O.ncol > 0;
+  const bool wormhole = A.S.kind == WORMHOLE;
+  double fs = 0, w[3];
+  bool sky = false;
+  if (wormhole) {  // skybox_handling! (LttM.jl:54)
+    double R[8];
+    for (int j = 0; j < 8; ++j) R[j] = P.geo[ray * 8 + j];
+    sky = skybox_texel(A.S, A.sky, R, fs, w);
+  }
   for (int c = 0; c < A.O.ncol; ++c) {
     const DevColor& k = A.colors[c];
-    const double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * A.O.ncol + c) * 2);
+    double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * A.O.ncol + c) * 2);
+    if (sky) {
+      const double f = k.f * fs;
+      v.x += skybox_I(w, f) * exp(-v.y) / (f * f * f);
+      if (A.out_rays) *reinterpret_cast<double2*>(P.spec + (ray * A.O.ncol + c) * 2) = v;
+    }
     double I = v.x * k.cx;
     X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
     if (!(v.y >= A.O.tau_thr)) thick = false;
@@ -198,7 +212,7 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   int cls;
   if (!isfinite(r)) cls = CLS_BOUNDARY;
   else if (A.S.kind != MINK && r <= A.S.r_horizon) cls = CLS_HORIZON;
-  else if (r > A.S.r_escape) cls = CLS_ESCAPE;
+  else if ((wormhole ? wh_r<double>(A.S, r) : r) > A.S.r_escape) cls = CLS_ESCAPE;
   else if (thick) cls = CLS_OPAQUE;
   else cls = CLS_BOUNDARY;
   if (A.out_xyz) { A.out_xyz[3 * ray] = X; A.out_xyz[3 * ray + 1] = Y; A.out_xyz[3 * ray + 2] = Z; }

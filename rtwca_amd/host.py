@@ -43,6 +43,7 @@ def _load():
     L.rtw_last_error.argtypes = [P]
     L.rtw_last_error.restype = C.c_char_p
     L.rtw_load_gaunt.argtypes = [P, C.c_char_p]
+    L.rtw_load_skybox.argtypes = [P, C.c_int32, P, C.c_int32, C.c_int32]
     L.rtw_render.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
     L.rtw_render_dev.argtypes = [P, P, P, C.c_int32, P, P, C.c_int32, P, P, P, P]
     L.rtw_xyz_to_srgb16.argtypes = [P, P, C.c_int32, C.c_int32, P]
@@ -93,6 +94,14 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def load_skybox(self, which, rgb):
+        """rtw_load_skybox: which = 1 (near side) or 2 (far side); rgb (3N, 4N, 3) uint8."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise RTWError("skybox must be an (3N, 4N, 3) uint8 array")
+        self._check(self.L.rtw_load_skybox(self.h, int(which), _ptr(rgb), rgb.shape[1], rgb.shape[0]),
+                    "rtw_load_skybox")
 
     def render(self, scene, cameras, integ, lam, init_rays=None, want_rays=False, want_class=True,
                want_steps=True):

@@ -160,6 +160,15 @@ def powehi_m87(ref_f32=True):
     return s
 
 
+def wormhole():
+    """main/Wormhole.source: the Dneg wormhole of Interstellar (rho = 1, half-length
+    0.005, lensing width 0.05; source_geometry.jl:7-13), no matter; rays that leave
+    through r(l) > 100 take the skybox of their side (needs both skyboxes loaded)."""
+    s = _base(abi.SCENE_WORMHOLE)
+    s.r_escape = 100.0
+    return s
+
+
 def kerr_synch(spin=0.94, r_in=6.0, r_max=12.0, density=1e45, jet_ratio=1e-4, jet_half_angle_deg=10.0,
                r_jet=10.0, bfield=10.0, theta_e=10.0):
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Fishbone-Moncrief
@@ -217,6 +226,8 @@ def initialize_world(name):
         return kerr_disk(), [0, 25, 0.52 * PI, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     if n == "m87":  # initialization.jl:100-108
         return powehi_m87(), [0, 35, 0.3, 0], [PI / 2, PI / 6], [0.0, 0.0, PI], 0.0
+    if n == "wormhole":  # initialization.jl:140-149
+        return wormhole(), [0, 5, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     if n == "minkowski_t,r,θ,ϕ_sphere":
         return minkowski_sphere(), [0, 4, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     raise ValueError("scene %r is not provided by rtwca_amd" % name)

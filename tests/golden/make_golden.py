@@ -1,6 +1,6 @@
 """Generate the committed fixtures under tests/golden/ from the oracle.
 
-    python tests/golden/make_golden.py [--flops-only | --synch-only | --m87-only]
+    python tests/golden/make_golden.py [--flops-only | --synch-only | --m87-only | --wormhole-only]
 
 * oracle_<cfg>_<npix>px_<nlam>[_<n>poses].npz: small renders of the benchmark workloads
   (XYZ, class, attempted/accepted steps, final states) -- the golden vectors that the
@@ -88,6 +88,35 @@ def make_m87_fixture():
     print("wrote", m87_fixture_name(), "steps", int(r["steps"][:, 0].sum()))
 
 
+WORMHOLE_FIXTURE = (24, 64)  # x_pix (FOV 85 x 60), skybox face size
+
+
+def wormhole_workload(x_pix=24, sky_n=64):
+    """The reference's wormhole test (initialization.jl:432-439: 350:30:740 nm, max_dt_scale
+    0.1, max_steps 1e4, gen_image's default tolerance 1e-4 and FOV) without ray bundles,
+    with synthetic skyboxes of face size sky_n."""
+    from rtwca_amd import abi, skybox
+    sc, pos, d, p, beta = scenes.initialize_world("wormhole")
+    cam = abi.make_camera(pos, d, beta, p, x_pix, 85, 60)
+    it = abi.make_integrator(1e-4, 1e-4, 0.1, 1e4)
+    lam = [float(x) for x in range(350, 751, 30)]
+    return sc, cam, it, lam, skybox.default_skyboxes(sky_n)
+
+
+def wormhole_fixture_name():
+    return os.path.join(HERE, "oracle_wormhole_%dpx_sky%d.npz" % WORMHOLE_FIXTURE)
+
+
+def make_wormhole_fixture():
+    sc, cam, it, lam, (s1, s2) = wormhole_workload(*WORMHOLE_FIXTURE)
+    O.load_skybox(1, s1)
+    O.load_skybox(2, s2)
+    r = O.render(sc, cam, it, lam, nthreads=os.cpu_count() or 1, want_rays=True)
+    np.savez_compressed(wormhole_fixture_name(), xyz=r["xyz"], cls=r["cls"], steps=r["steps"], rays=r["rays"],
+                        lam=np.array(lam))
+    print("wrote", wormhole_fixture_name(), "steps", int(r["steps"][:, 0].sum()))
+
+
 def make_flops():
     out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
                          "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
@@ -122,8 +151,12 @@ if __name__ == "__main__":
     elif "--m87-only" in sys.argv:
         make_m87_fixture()
         sys.exit(0)
+    elif "--wormhole-only" in sys.argv:
+        make_wormhole_fixture()
+        sys.exit(0)
     elif "--flops-only" not in sys.argv:
         make_fixtures()
         make_synch_fixture()
         make_m87_fixture()
+        make_wormhole_fixture()
     make_flops()
