@@ -161,7 +161,11 @@ typedef struct rtw_integrator {
   int32_t schedule;       /* rtw_schedule */
   double fp64_radius;     /* MIXED: steps starting at r < fp64_radius run in FP64 */
   int32_t lanes_per_ray;  /* 0 = library default */
-  int32_t reserved;
+  int32_t ray_bundles;    /* gen_image's ray_bundles (initialization.jl:210-211, 246-324): 11
+                             constant bundle slots follow the colours (out_rays rows are
+                             8 + 2N + 11), the wormhole's skybox is sampled over the beam
+                             (Wormhole.source/source_model.jl:291-389) and, as in the
+                             reference's calc_terminate, rays no longer stop for opacity */
 } rtw_integrator;
 
 /* Termination class per pixel (the reference stops without recording why;

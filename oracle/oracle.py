@@ -64,6 +64,11 @@ def lib():
         L.rtwo_skybox_I.argtypes = [C.c_int, C.c_int, C.c_int, D]
         L.rtwo_skybox_I.restype = D
         L.rtwo_wormhole_r.argtypes = [D, P]
+        L.rtwo_bundle_params.argtypes = [P, C.c_int, C.c_int, P]
+        L.rtwo_cube_wrap.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]
+        L.rtwo_distort_gauss.argtypes = [D, D, D, D, D, P]
+        L.rtwo_d2_gauss.argtypes = [D, D, D, D, D]
+        L.rtwo_d2_gauss.restype = D
         L.rtwo_powehi_consts.argtypes = [P, P]
         rc = L.rtwo_load_gaunt(GAUNT_PATH.encode())
         if rc != 0:
@@ -143,6 +148,29 @@ def skybox_coord(ray8, N):
 
 def skybox_I(which, ia, ja, f):
     return lib().rtwo_skybox_I(int(which), int(ia), int(ja), float(f))
+
+
+def bundle_params(cam, i, j):
+    """The 11 ray-bundle slots of pixel (i, j) (0-based)."""
+    out = np.zeros(11)
+    lib().rtwo_bundle_params(_r(cam), int(i), int(j), _p(out))
+    return out
+
+
+def cube_wrap(i, j, face, x, y, N):
+    out = np.zeros(3, dtype=np.int32)
+    lib().rtwo_cube_wrap(int(i), int(j), int(face), int(x), int(y), int(N), _p(out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def distort_gauss(sx, sy, th, a, b):
+    out = np.zeros(3)
+    lib().rtwo_distort_gauss(sx, sy, th, a, b, _p(out))
+    return tuple(out)
+
+
+def d2_gauss(x, y, sx, sy, th):
+    return lib().rtwo_d2_gauss(float(x), float(y), sx, sy, th)
 
 
 def wormhole_r(l):
@@ -291,7 +319,7 @@ def render(scene, cam, integ, lam, nthreads=1, subset=None, init_rays=None, want
     lam = np.ascontiguousarray(lam, dtype=np.float64)
     ny = camera_y_pix(cam)
     npix = cam.x_pix * ny
-    n = 8 + 2 * len(lam)
+    n = 8 + 2 * len(lam) + (11 if getattr(integ, "ray_bundles", 0) else 0)
     xyz = np.zeros((npix, 3))
     cls = np.zeros(npix, dtype=np.uint8)
     steps = np.zeros((npix, 2), dtype=np.int64)

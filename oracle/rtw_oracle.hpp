@@ -902,6 +902,8 @@ struct Opts {
   double rtol = 1e-8, atol = 1e-10;
   double max_dt_scale = 0.1;
   double max_steps = 1e4;
+  bool bundles = false;  // ray_bundles: 11 constant slots after the colours
+  double bundle_du = 0, bundle_dg = 0;  // this ray's du, dg (the non-zero even bundle slots)
 };
 
 // Termination classes (defined by this build; the reference stops without saying why).
@@ -979,6 +981,10 @@ template <class R> struct RaySys {
     R thr(-std::log(o.atol));
     for (int i = 9; i < n; i += 2)
       if (ray[i] < thr) return false;
+    // calc_terminate's loop 10:2:raylength also reads the even bundle slots
+    // (v, h, du, dg, dchi = 0, 0, du, dg, 0): with bundles a ray never stops as opaque
+    // while -log(atol) > 0
+    if (o.bundles && (R(0) < thr || R(o.bundle_du) < thr || R(o.bundle_dg) < thr)) return false;
     return true;
   }
   int classify(const R* ray) const {
@@ -1124,7 +1130,11 @@ inline double normal_gauss(double x, double sg, double mu) {
 // skybox_I_nu_calc (:150-172): RGB of the texel at absolute cube-map (i, j) (1-based;
 // row 3N + 1 - j, column i) spread into Gaussian lines at 630, 532 and 467 nm.
 inline double skybox_I_nu(const Skybox& sb, int i, int j, double f) {
-  const size_t row = (size_t)(3 * sb.N - j), col = (size_t)(i - 1);
+  // absolute coordinates outside the image (cube_wrap leaving a face; the reference
+  // raises a BoundsError there) are clamped to the image
+  const long long r0 = std::min(std::max(3LL * sb.N - j, 0LL), 3LL * sb.N - 1);
+  const long long c0 = std::min(std::max((long long)i - 1, 0LL), 4LL * sb.N - 1);
+  const size_t row = (size_t)r0, col = (size_t)c0;
   const uint8_t* px = &sb.rgb[(row * 4 * sb.N + col) * 3];
   const double R = 1.0 * (px[0] / 255.0), G = 0.843 * (px[1] / 255.0), B = 0.914 * (px[2] / 255.0);
   return 1e-12 * (R * normal_gauss(f, 5e-5, 0.001587) + G * normal_gauss(f, 7e-5, 0.001879) +
@@ -1403,6 +1413,231 @@ inline void camera_ray(const Camera& c, const M4 Mc, int i, int j, int y_pix, do
     for (int b = 0; b < 4; ++b) sacc += Mc[a][b] * v[b];
     ray8[4 + a] = sacc;
   }
+}
+
+// Ray-bundle parameters of pixel (i, j) (initialization.jl:202-205, 246-249, 263-324):
+// out11 = (u, v, g, h, chi, du, dv, dg, dh, dchi, delta), the last 11 slots of the ray.
+// Their slopes are zero (integrators_LttM.jl:44-46), so they keep these values.
+inline void bundle_params(const Camera& c, int i, int j, int y_pix, double* out11) {
+  double fh = PI * c.fov_hor / 180, fv = PI * c.fov_vert / 180;
+  double W = std::tan(fh / 2), H = std::tan(fv / 2);
+  double rh = 2 * W / (c.x_pix + 1), rv = 2 * H / (y_pix + 1);
+  const double beamscale = std::atan(rh) / 2;
+  double w = rh * ((i + 1) - 0.5) - W;
+  double h = rv * ((j + 1) - 0.5) - H;
+  double th = std::atan(std::sqrt(w * w + h * h));
+  double ph;
+  if (w > 0) ph = std::atan(h / w);
+  else if (w < 0 && h >= 0) ph = std::atan(h / w) + PI;
+  else if (w < 0 && h < 0) ph = std::atan(h / w) - PI;
+  else if (w == 0 && h > 0) ph = PI / 2;
+  else if (w == 0 && h < 0) ph = -PI / 2;
+  else ph = 0;
+  double n[3] = {-std::sin(th) * std::cos(ph), -std::sin(th) * std::sin(ph), -std::cos(th)};
+  const double delta = std::fabs(beamscale * n[2]);
+  const double E = 1 / (std::fabs(n[2]) + NO_DIV_ZERO);
+  double J[3] = {-n[1], n[0], 0.0};  // cross(n, [0, 0, -1])
+  if (J[0] * J[0] + J[1] * J[1] + J[2] * J[2] < 1e-3) { J[0] = 0; J[1] = -1; J[2] = 0; }
+  // R2[2:4,2:4] L[2:4,2:4] R[2:4,2:4], applied to J and n
+  M4 R, L, R2;
+  intrinsic_rotation(c.point, R);
+  boost_z(-c.beta, L);
+  final_rotation(c.dir, R2);
+  double A3[3][3], B3[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double sacc = 0;
+      for (int k = 0; k < 3; ++k) sacc += R2[1 + a][1 + k] * L[1 + k][1 + b];
+      A3[a][b] = sacc;
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double sacc = 0;
+      for (int k = 0; k < 3; ++k) sacc += A3[a][k] * R[1 + k][1 + b];
+      B3[a][b] = sacc;
+    }
+  double Jr[3], nr[3];
+  for (int a = 0; a < 3; ++a) {
+    Jr[a] = B3[a][0] * J[0] + B3[a][1] * J[1] + B3[a][2] * J[2];
+    nr[a] = B3[a][0] * n[0] + B3[a][1] * n[1] + B3[a][2] * n[2];
+  }
+  const double jn = std::sqrt(Jr[0] * Jr[0] + Jr[1] * Jr[1] + Jr[2] * Jr[2]);
+  const double nn = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+  for (int a = 0; a < 3; ++a) { Jr[a] /= jn; nr[a] /= nn; }
+  double av[3], bv[3];
+  if (!(std::fabs(1 - nr[0]) < 1e-3)) {
+    av[0] = nr[1]; av[1] = 1 - nr[1] * nr[1] / (1 - nr[0]); av[2] = -nr[1] * nr[2] / (1 - nr[0]);
+    bv[0] = nr[2]; bv[1] = -nr[1] * nr[2] / (1 - nr[0]); bv[2] = 1 - nr[2] * nr[2] / (1 - nr[0]);
+    // the reference divides by the squared norm (a = a / dot(a, a))
+    const double da = av[0] * av[0] + av[1] * av[1] + av[2] * av[2];
+    const double db = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
+    for (int a = 0; a < 3; ++a) { av[a] /= da; bv[a] /= db; }
+  } else {
+    av[0] = 0; av[1] = -1; av[2] = 0;
+    bv[0] = 0; bv[1] = 0; bv[2] = -1;
+  }
+  const double aJ = av[0] * Jr[0] + av[1] * Jr[1] + av[2] * Jr[2];
+  const double bJ = bv[0] * Jr[0] + bv[1] * Jr[1] + bv[2] * Jr[2];
+  for (int k = 0; k < 5; ++k) out11[k] = 0;
+  out11[5] = aJ * (1 + E) / 2;
+  out11[6] = bJ * (1 + E) / 2;
+  out11[7] = aJ * (1 - E) / 2;
+  out11[8] = bJ * (1 - E) / 2;
+  out11[9] = 0;
+  out11[10] = delta;
+}
+
+// Julia mod1(x, m) for m > 0: the representative in 1..m.
+inline int jmod1(int x, int m) { int r = (x - 1) % m; if (r < 0) r += m; return r + 1; }
+
+// cube_wrap (gen_geometry.jl:128-345): texel (i, j) of face F shifted by (x, y) texels,
+// following the shift across face edges.  A diagonal shift is taken as x then y (with
+// the order and orientation the reference uses on faces 5 and 6).
+inline void cube_wrap(int i1, int j1, int F1, int x, int y, int N, int& i2, int& j2, int& F2) {
+  i2 = i1; j2 = j1; F2 = F1;
+  if (!(x != 0 && y != 0)) {
+    switch (F1) {
+      case 1:
+        if (x != 0) {
+          if (i1 + x > N) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 2; }
+          else if (i1 + x < 1) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 4; }
+          else { i2 = i1 + x; j2 = j1; F2 = 1; }
+        } else {
+          if (j1 + y > N) { i2 = jmod1(j1 + y, N); j2 = N + 1 - i1; F2 = 6; }
+          else if (j1 + y < 1) { i2 = 1 - j1 - y; j2 = i1; F2 = 5; }
+          else { i2 = i1; j2 = j1 + y; F2 = 1; }
+        }
+        break;
+      case 2:
+        if (x != 0) {
+          if (i1 + x > N) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 3; }
+          else if (i1 + x < 1) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 1; }
+          else { i2 = i1 + x; j2 = j1; F2 = 2; }
+        } else {
+          if (j1 + y > N) { i2 = i1; j2 = jmod1(j1 + y, N); F2 = 6; }
+          else if (j1 + y < 1) { i2 = i1; j2 = jmod1(j1 + y, N); F2 = 5; }
+          else { i2 = i1; j2 = j1 + y; F2 = 2; }
+        }
+        break;
+      case 3:
+        if (x != 0) {
+          if (i1 + x > N) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 4; }
+          else if (i1 + x < 1) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 2; }
+          else { i2 = i1 + x; j2 = j1; F2 = 3; }
+        } else {
+          if (j1 + y > N) { i2 = N - jmod1(j1 + y, N) + 1; j2 = i1; F2 = 6; }
+          else if (j1 + y < 1) { i2 = N + j1 + y; j2 = N - i1 + 1; F2 = 5; }
+          else { i2 = i1; j2 = j1 + y; F2 = 3; }
+        }
+        break;
+      case 4:
+        if (x != 0) {
+          if (i1 + x > N) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 1; }
+          else if (i1 + x < 1) { i2 = jmod1(i1 + x, N); j2 = j1; F2 = 3; }
+          else { i2 = i1 + x; j2 = j1; F2 = 4; }
+        } else {
+          if (j1 + y > N) { i2 = N - i1 + 1; j2 = N - jmod1(j1 + y, N) + 1; F2 = 6; }
+          else if (j1 + y < 1) { i2 = N - i1 + 1; j2 = 1 - j1 - y; F2 = 5; }
+          else { i2 = i1; j2 = j1 + y; F2 = 4; }
+        }
+        break;
+      case 5:
+        if (x != 0) {
+          if (i1 + x > N) { i2 = N - j1 + 1; j2 = jmod1(i1 + x, N); F2 = 3; }
+          else if (i1 + x < 1) { i2 = j1; j2 = 1 - i1 - x; F2 = 1; }
+          else { i2 = i1 + x; j2 = j1; F2 = 5; }
+        } else {
+          if (j1 + y > N) { i2 = i1; j2 = jmod1(j1 + y, N); F2 = 2; }
+          else if (j1 + y < 1) { i2 = N - i1 + 1; j2 = 1 - j1 - y; F2 = 4; }
+          else { i2 = i1; j2 = j1 + y; F2 = 5; }
+        }
+        break;
+      default:  // 6
+        if (x != 0) {
+          if (i1 + x > N) { i2 = j1; j2 = N - jmod1(i1 + x, N) + 1; F2 = 3; }
+          else if (i1 + x < 1) { i2 = N - j1 + 1; j2 = N - i1 - x; F2 = 1; }
+          else { i2 = i1 + x; j2 = j1; F2 = 6; }
+        } else {
+          if (j1 + y > N) { i2 = N - i1 + 1; j2 = N - jmod1(j1 + y, N) + 1; F2 = 4; }
+          else if (j1 + y < 1) { i2 = i1; j2 = jmod1(j1 + y, N); F2 = 2; }
+          else { i2 = i1; j2 = j1 + y; F2 = 6; }
+        }
+        break;
+    }
+    return;
+  }
+  int ia, ja, Fa;
+  if (F1 == 6) {
+    cube_wrap(i1, j1, F1, x, 0, N, ia, ja, Fa);
+    if (i1 + x < 1) cube_wrap(ia, ja, Fa, -y, 0, N, i2, j2, F2);
+    else if (i1 + x > N) cube_wrap(ia, ja, Fa, y, 0, N, i2, j2, F2);
+    else cube_wrap(ia, ja, Fa, 0, y, N, i2, j2, F2);
+  } else if (F1 == 5) {
+    cube_wrap(i1, j1, F1, x, 0, N, ia, ja, Fa);
+    if (i1 + x > N) cube_wrap(ia, ja, Fa, -y, 0, N, i2, j2, F2);
+    else if (i1 + x < 1) cube_wrap(ia, ja, Fa, y, 0, N, i2, j2, F2);
+    else cube_wrap(ia, ja, Fa, 0, y, N, i2, j2, F2);
+  } else {
+    cube_wrap(i1, j1, F1, x, 0, N, ia, ja, Fa);
+    cube_wrap(ia, ja, Fa, 0, y, N, i2, j2, F2);
+  }
+}
+
+// distort_gauss and D2_Gauss (main/my_special_functions.jl:28-55)
+inline void distort_gauss(double sx, double sy, double th, double a, double b, double& sxn, double& syn,
+                          double& thn) {
+  const double c = std::cos(th), s = std::sin(th);
+  const double Sxx = sq(a) * (sq(sx) * sq(c) + sq(sy) * sq(s));
+  const double Syy = sq(b) * (sq(sx) * sq(s) + sq(sy) * sq(c));
+  const double Sxy = a * b * (sq(sx) - sq(sy)) * std::sin(2 * th) / 2;
+  const double rt = std::sqrt(sq(Sxx - Syy) + 4 * sq(Sxy));
+  sxn = std::sqrt((Sxx + Syy + rt) / 2);
+  syn = std::sqrt((Sxx + Syy - rt) / 2);
+  thn = std::atan2(2 * Sxy, Sxx - Syy) / 2;
+}
+inline double d2_gauss(double x, double y, double sx, double sy, double th) {
+  const double c = std::cos(th), s = std::sin(th);
+  const double a = sq(c) / (2 * sq(sx)) + sq(s) / (2 * sq(sy));
+  const double b = -s * c / (2 * sq(sx)) + s * c / (2 * sq(sy));
+  const double cc = sq(s) / (2 * sq(sx)) + sq(c) / (2 * sq(sy));
+  return (1 / (2 * PI * sx * sy)) * std::exp(-(a * sq(x) + 2 * b * x * y + cc * sq(y)));
+}
+
+// skybox_handling! with ray bundles (Wormhole.source/source_model.jl:291-389, 393-443):
+// a Gaussian-weighted (2h+1)^2 texel neighbourhood, the beam shape from the bundle slots.
+inline void skybox_handling_beam(const Scene& s, double* ray, int ncol, const double* freq, const double* b11) {
+  if (s.kind != WORMHOLE) return;
+  const double rl = wh_r(ray[1]);
+  if (!(rl > 50) || ray[1] == 0) return;
+  const Skybox& sb = skybox(ray[1] > 0 ? 1 : 2);
+  const int N = sb.N;
+  const double fs = ray[4];
+  int i, j, face;
+  double u, v;
+  skybox_coord(ray, N, i, j, u, v, face);
+  const double dp = b11[10] * fs;
+  const double m1 = std::sqrt(sq(b11[5]) + sq(b11[6])), m2 = std::sqrt(sq(b11[7]) + sq(b11[8]));
+  const double dmaj = dp * (m1 + m2), dmin = dp * (m1 - m2);
+  // angle(complex(du, dv) * complex(dg, dh)) / 2
+  const double re = b11[5] * b11[7] - b11[6] * b11[8], im = b11[5] * b11[8] + b11[6] * b11[7];
+  const double mu = b11[4] + std::atan2(im, re) / 2;
+  const double sbs = std::atan(2.0 / (N + 1)) / 2;
+  double sxn, syn, mun;
+  distort_gauss(dmaj, dmin, mu, 1 / (sbs + NO_DIV_ZERO), 1 / (sbs + NO_DIV_ZERO), sxn, syn, mun);
+  const int h = std::min((int)std::ceil(std::max(sxn, syn)), 10);
+  std::vector<double> I(ncol, 0.0);
+  for (int k = -h; k <= h; ++k)
+    for (int l = -h; l <= h; ++l) {
+      int iw, jw, fw, ia, ja;
+      cube_wrap(i, j, face, k, l, N, iw, jw, fw);
+      cubemap_abs(iw, jw, fw, N, ia, ja);
+      const double wgt = d2_gauss(k, l, sxn, syn, mun);
+      for (int c = 0; c < ncol; ++c) {
+        const double f = freq[c] * fs;
+        I[c] += wgt * skybox_I_nu(sb, ia, ja, f) * std::exp(-ray[9 + 2 * c]) / cube(f);
+      }
+    }
+  for (int c = 0; c < ncol; ++c) ray[8 + 2 * c] += I[c];
 }
 
 // is_singularity (Kerr source_geometry.jl:307-315 etc.): camera validity guard.

@@ -38,6 +38,7 @@ static Camera to_cam(const rtw_camera* c) {
 static Opts to_opts(const rtw_integrator* p) {
   Opts o;
   o.rtol = p->rtol; o.atol = p->atol; o.max_dt_scale = p->max_dt_scale; o.max_steps = p->max_steps;
+  o.bundles = p->ray_bundles != 0;
   return o;
 }
 
@@ -175,6 +176,21 @@ void rtwo_skybox_coord(const double* ray8, int N, double* out7) {
   out7[0] = i; out7[1] = j; out7[2] = u; out7[3] = v; out7[4] = face; out7[5] = ia; out7[6] = ja;
 }
 double rtwo_skybox_I(int which, int ia, int ja, double f) { return skybox_I_nu(skybox(which), ia, ja, f); }
+// Ray-bundle slots of pixel (i, j) of a camera; cube_wrap of (i, j, face) by (x, y).
+void rtwo_bundle_params(const rtw_camera* c, int i, int j, double* out11) {
+  Camera cam = to_cam(c);
+  bundle_params(cam, i, j, camera_y_pix(cam), out11);
+}
+void rtwo_cube_wrap(int i, int j, int face, int x, int y, int N, int* out3) {
+  cube_wrap(i, j, face, x, y, N, out3[0], out3[1], out3[2]);
+}
+
+// Beam shape in texels (distort_gauss of the skybox beam) and a D2_Gauss weight.
+void rtwo_distort_gauss(double sx, double sy, double th, double a, double b, double* out3) {
+  distort_gauss(sx, sy, th, a, b, out3[0], out3[1], out3[2]);
+}
+double rtwo_d2_gauss(double x, double y, double sx, double sy, double th) { return d2_gauss(x, y, sx, sy, th); }
+
 // Wormhole r(l) and the reference's dr_dl.
 void rtwo_wormhole_r(double l, double* out2) { out2[0] = wh_r<double>(l); out2[1] = wh_drdl<double>(l); }
 
@@ -305,6 +321,7 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
   Opts op = to_opts(o);
   int ny = camera_y_pix(cam);
   int64_t np = (int64_t)cam.x_pix * ny;
+  const int nb = op.bundles ? 11 : 0;
   int n = 8 + 2 * nlam;
   std::vector<double> freq(nlam), WX(nlam), WY(nlam), WZ(nlam);
   for (int i = 0; i < nlam; ++i) freq[i] = 1.0 / lam[i];
@@ -331,15 +348,26 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
         } else {
           camera_ray(cam, M, (int)(p % cam.x_pix), (int)(p / cam.x_pix), ny, ray.data());
         }
-        RaySys<double> sys(&sc, op, nlam, freq.data());
+        double b11[11] = {0};
+        Opts opr = op;
+        if (nb) {
+          bundle_params(cam, (int)(p % cam.x_pix), (int)(p / cam.x_pix), ny, b11);
+          opr.bundle_du = b11[5];
+          opr.bundle_dg = b11[7];
+        }
+        RaySys<double> sys(&sc, opr, nlam, freq.data());
         RayStats st = integrate_ray(sys, ray.data());
-        skybox_handling(sc, ray.data(), nlam, freq.data());  // LttM.jl:54
+        if (nb) skybox_handling_beam(sc, ray.data(), nlam, freq.data(), b11);  // LttM.jl:54
+        else skybox_handling(sc, ray.data(), nlam, freq.data());
         double xyz[3];
         calc_xyz(ray.data(), lam, freq.data(), nlam, WX.data(), WY.data(), WZ.data(), xyz);
         if (out_xyz) for (int k = 0; k < 3; ++k) out_xyz[3 * p + k] = xyz[k];
         if (out_cls) out_cls[p] = (uint8_t)st.cls;
         if (out_steps) { out_steps[2 * p] = st.attempted; out_steps[2 * p + 1] = st.accepted; }
-        if (out_rays) for (int k = 0; k < n; ++k) out_rays[p + k * np] = ray[k];
+        if (out_rays) {
+          for (int k = 0; k < n; ++k) out_rays[p + k * np] = ray[k];
+          for (int k = 0; k < nb; ++k) out_rays[p + (n + k) * np] = b11[k];
+        }
       }
     }
   };

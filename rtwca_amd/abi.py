@@ -63,7 +63,7 @@ class Integrator(C.Structure):
     _fields_ = [
         ("rtol", C.c_double), ("atol", C.c_double), ("max_dt_scale", C.c_double),
         ("max_steps", C.c_double), ("precision", C.c_int32), ("schedule", C.c_int32),
-        ("fp64_radius", C.c_double), ("lanes_per_ray", C.c_int32), ("reserved", C.c_int32),
+        ("fp64_radius", C.c_double), ("lanes_per_ray", C.c_int32), ("ray_bundles", C.c_int32),
     ]
 
 
@@ -94,7 +94,7 @@ def make_camera(pos, direction=(0.0, 0.0), beta=0.0, pointing=(0.0, 0.0, 0.0),
 
 def make_integrator(rtol=1e-4, atol=None, max_dt_scale=0.1, max_steps=1e3,
                     precision=PREC_FP64, schedule=SCHED_PERSISTENT, fp64_radius=3.0,
-                    lanes_per_ray=0):
+                    lanes_per_ray=0, ray_bundles=False):
     o = Integrator()
     o.rtol = float(rtol)
     o.atol = float(rtol if atol is None else atol)
@@ -104,6 +104,7 @@ def make_integrator(rtol=1e-4, atol=None, max_dt_scale=0.1, max_steps=1e3,
     o.schedule = int(schedule)
     o.fp64_radius = float(fp64_radius)
     o.lanes_per_ray = int(lanes_per_ray)
+    o.ray_bundles = 1 if ray_bundles else 0
     return o
 
 

@@ -219,6 +219,21 @@ static DevCam make_cam(const DevScene& S, const rtw_camera& c) {
   d.H = std::tan(M_PI * c.fov_vert / 180 / 2);
   d.rho_h = 2 * d.W / (c.x_pix + 1);
   d.rho_v = 2 * d.H / (ny + 1);
+  d.beamscale = std::atan(d.rho_h) / 2;
+  // R2[2:4,2:4] L[2:4,2:4] R[2:4,2:4] for the ray-bundle axes (initialization.jl:272-273)
+  double A3[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += R2[1 + a][1 + k] * L[1 + k][1 + b];
+      A3[a][b] = s;
+    }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += A3[a][k] * R[1 + k][1 + b];
+      d.B3[a * 3 + b] = s;
+    }
   return d;
 }
 
@@ -626,6 +641,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
     return fail(c, RTW_E_ARG, "invalid tolerances/max_dt_scale/max_steps");
   if (opts->precision != RTW_PREC_FP64 && opts->precision != RTW_PREC_MIXED)
     return fail(c, RTW_E_ARG, "unknown precision");
+  if (opts->ray_bundles && !(opts->atol < 1)) return fail(c, RTW_E_ARG, "ray bundles need atol < 1");
   for (int k = 0; k < n_lambda; ++k)
     if (!(lam[k] > 0)) return fail(c, RTW_E_ARG, "wavelengths must be positive");
   const rtw_camera& c0 = cams[0];
@@ -676,6 +692,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   A.O.fp64_radius = opts->fp64_radius;
   A.O.ncol = n_lambda;
   A.O.n = 8 + 2 * n_lambda;
+  A.O.bundles = opts->ray_bundles ? 1 : 0;
   A.cams = c->cams.as<DevCam>();
   A.x_pix = c0.x_pix;
   A.y_pix = ny;
@@ -829,7 +846,7 @@ int rtw_render(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cams, int32
   if (!out_xyz || !cams || n_cams < 1) return fail(c, RTW_E_ARG, "NULL or invalid argument");
   int ny = cam_y_pix(cams[0]);
   long long total = (long long)cams[0].x_pix * (ny > 0 ? ny : 0) * n_cams;
-  int n = 8 + 2 * (n_lambda > 0 ? n_lambda : 0);
+  int n = 8 + 2 * (n_lambda > 0 ? n_lambda : 0) + (opts && opts->ray_bundles ? 11 : 0);
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, c->xyz.ensure(sizeof(double) * 3 * (total > 0 ? total : 1)));
   if (out_class) HIPCHK(c, c->cls.ensure(total > 0 ? total : 1));

@@ -67,6 +67,7 @@ struct DevScene {
 struct DevOpts {
   double rtol, atol, max_steps, tau_thr, fp64_radius;
   int ncol, n;
+  int bundles;  // ray bundles: no opacity stop, beam-sampled skybox, 11 more out_rays rows
 };
 
 // Per-colour constants (staged in LDS).
@@ -84,6 +85,8 @@ struct DevCam {
   double Mc[16];
   double pos[4];
   double W, H, rho_h, rho_v;
+  double B3[9];       // spatial part of R2 L R (ray bundles, initialization.jl:272-273)
+  double beamscale;   // atan(rho_h) / 2 (initialization.jl:205)
 };
 
 template <class T> __device__ __forceinline__ T sq(T x) { return x * x; }
@@ -119,6 +122,50 @@ __device__ __forceinline__ void camera_ray(const DevCam& C, long long pix, int x
     for (int b = 0; b < 4; ++b) s += C.Mc[a * 4 + b] * v[b];
     R[4 + a] = s;
   }
+}
+
+// Ray-bundle slots of a pixel (initialization.jl:246-249, 263-324): (u, v, g, h, chi,
+// du, dv, dg, dh, dchi, delta); constant along the ray (zero slopes,
+// integrators_LttM.jl:44-46), so they are formed where they are needed.
+__device__ __forceinline__ void bundle_slots(const DevCam& C, long long pix, int x_pix, double b[11]) {
+  double v[4];
+  camera_local(C, pix, x_pix, v);
+  const double n0 = v[1], n1 = v[2], n2 = v[3];
+  const double E = 1.0 / (fabs(n2) + NO_DIV_ZERO);
+  double J[3] = {-n1, n0, 0.0};  // cross(n, [0, 0, -1])
+  if (J[0] * J[0] + J[1] * J[1] + J[2] * J[2] < 1e-3) { J[0] = 0; J[1] = -1; J[2] = 0; }
+  double Jr[3], nr[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    Jr[a] = C.B3[a * 3] * J[0] + C.B3[a * 3 + 1] * J[1] + C.B3[a * 3 + 2] * J[2];
+    nr[a] = C.B3[a * 3] * n0 + C.B3[a * 3 + 1] * n1 + C.B3[a * 3 + 2] * n2;
+  }
+  const double jn = sqrt(Jr[0] * Jr[0] + Jr[1] * Jr[1] + Jr[2] * Jr[2]);
+  const double nn = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { Jr[a] /= jn; nr[a] /= nn; }
+  double av[3], bv[3];
+  if (!(fabs(1 - nr[0]) < 1e-3)) {
+    av[0] = nr[1]; av[1] = 1 - nr[1] * nr[1] / (1 - nr[0]); av[2] = -nr[1] * nr[2] / (1 - nr[0]);
+    bv[0] = nr[2]; bv[1] = -nr[1] * nr[2] / (1 - nr[0]); bv[2] = 1 - nr[2] * nr[2] / (1 - nr[0]);
+    // divided by the squared norm, as the reference does
+    const double da = av[0] * av[0] + av[1] * av[1] + av[2] * av[2];
+    const double db = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { av[a] /= da; bv[a] /= db; }
+  } else {
+    av[0] = 0; av[1] = -1; av[2] = 0;
+    bv[0] = 0; bv[1] = 0; bv[2] = -1;
+  }
+  const double aJ = av[0] * Jr[0] + av[1] * Jr[1] + av[2] * Jr[2];
+  const double bJ = bv[0] * Jr[0] + bv[1] * Jr[1] + bv[2] * Jr[2];
+  b[0] = b[1] = b[2] = b[3] = b[4] = 0;
+  b[5] = aJ * (1 + E) / 2;
+  b[6] = bJ * (1 + E) / 2;
+  b[7] = aJ * (1 - E) / 2;
+  b[8] = bJ * (1 - E) / 2;
+  b[9] = 0;
+  b[10] = fabs(C.beamscale * n2);
 }
 
 __device__ __forceinline__ void sc_(double x, double& s, double& c) { sincos(x, &s, &c); }
@@ -704,12 +751,10 @@ struct DevSky {
   const uint8_t* rgb[2];  // [3N][4N][3] each, row-major
   int N[2];
 };
-__device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K, const double* R, double& fs,
-                                             double w[3]) {
+// Face and in-face texel (1-based) of the direction R_sph_to_cart e k
+// (calc_skybox_coord); false for a non-finite direction.
+__device__ __forceinline__ bool skybox_face(const DevScene& S, const double* R, int N, int& face, int& i, int& j) {
   const double rl = wh_r<double>(S, R[1]);
-  if (!(rl > 50) || R[1] == 0) return false;
-  const int b = R[1] > 0 ? 0 : 1;
-  const int N = K.N[b];
   double st, ct, sp, cp;
   sincos(R[2], &st, &ct);
   sincos(R[3], &sp, &cp);
@@ -720,7 +765,6 @@ __device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K,
   const double nrm = sqrt(vx * vx + vy * vy + vz * vz);
   const double x = vx / nrm, y = vy / nrm, z = vz / nrm;
   const double ax = fabs(x), ay = fabs(y), az = fabs(z);
-  int face;
   double u, v;
   if (ax >= ay && ax >= az) face = x > 0 ? 1 : 3;
   else if (ay >= ax && ay >= az) face = y > 0 ? 4 : 2;
@@ -734,19 +778,140 @@ __device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K,
     default: u = -x / az; v = y / az; break;
   }
   if (!(fabs(u) <= 1.0) || !(fabs(v) <= 1.0)) return false;  // non-finite direction: no texel
-  int i = (int)floor(N * (u + 1) / 2) + 1, j = (int)floor(N * (v + 1) / 2) + 1;
+  i = (int)floor(N * (u + 1) / 2) + 1;
+  j = (int)floor(N * (v + 1) / 2) + 1;
   i = i < N ? i : N;
   j = j < N ? j : N;
-  // cubemap_coord_calc: face offsets in the horizontal cross
+  return true;
+}
+// Weighted RGB of texel (i, j) of a face (cubemap_coord_calc + skybox_I_nu_calc's
+// channel weights); (i, j) in 1..N, face in 1..6.
+__device__ __forceinline__ void skybox_rgb(const uint8_t* rgb, int N, int face, int i, int j, double w[3]) {
   const int di = (face == 2 || face == 5 || face == 6) ? 1 : (face == 3 ? 2 : (face == 4 ? 3 : 0));
   const int dj = (face == 5) ? 0 : (face == 6 ? 2 : 1);
-  const long long row = 3LL * N - (j + dj * N), col = (long long)(i + di * N) - 1;
-  const uint8_t* px = K.rgb[b] + (row * 4LL * N + col) * 3;
+  long long row = 3LL * N - (j + dj * N), col = (long long)(i + di * N) - 1;
+  // cube_wrap can leave the face (multi-texel shifts out of face 6 to the left, or N <
+  // 10): the reference then reads that pixel of its image, or raises a BoundsError
+  // outside the image; the index is kept inside the image
+  row = row < 0 ? 0 : (row > 3LL * N - 1 ? 3LL * N - 1 : row);
+  col = col < 0 ? 0 : (col > 4LL * N - 1 ? 4LL * N - 1 : col);
+  const uint8_t* px = rgb + (row * 4LL * N + col) * 3;
   w[0] = 1.0 * (px[0] / 255.0);
   w[1] = 0.843 * (px[1] / 255.0);
   w[2] = 0.914 * (px[2] / 255.0);
+}
+__device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K, const double* R, double& fs,
+                                             double w[3]) {
+  if (!(wh_r<double>(S, R[1]) > 50) || R[1] == 0) return false;
+  const int b = R[1] > 0 ? 0 : 1;
+  int face, i, j;
+  if (!skybox_face(S, R, K.N[b], face, i, j)) return false;
+  skybox_rgb(K.rgb[b], K.N[b], face, i, j, w);
   fs = R[4];
   return true;
+}
+
+// Julia mod1(x, m), m > 0
+__device__ __forceinline__ int jmod1(int x, int m) {
+  int r = (x - 1) % m;
+  if (r < 0) r += m;
+  return r + 1;
+}
+// cube_wrap for a shift along one axis (gen_geometry.jl:128-278): the texel (i, j) of
+// face F moved by x (or y) texels, continued across the face edge.
+__device__ __forceinline__ void cube_wrap1(int i1, int j1, int F1, int x, int y, int N, int& i2, int& j2, int& F2) {
+  i2 = i1; j2 = j1; F2 = F1;
+  const bool hx = x != 0;
+  const int ix = i1 + x, jy = j1 + y;
+  switch (F1) {
+    case 1:
+      if (hx) { if (ix > N) { i2 = jmod1(ix, N); F2 = 2; } else if (ix < 1) { i2 = jmod1(ix, N); F2 = 4; } else i2 = ix; }
+      else { if (jy > N) { i2 = jmod1(jy, N); j2 = N + 1 - i1; F2 = 6; } else if (jy < 1) { i2 = 1 - jy; j2 = i1; F2 = 5; } else j2 = jy; }
+      break;
+    case 2:
+      if (hx) { if (ix > N) { i2 = jmod1(ix, N); F2 = 3; } else if (ix < 1) { i2 = jmod1(ix, N); F2 = 1; } else i2 = ix; }
+      else { if (jy > N) { j2 = jmod1(jy, N); F2 = 6; } else if (jy < 1) { j2 = jmod1(jy, N); F2 = 5; } else j2 = jy; }
+      break;
+    case 3:
+      if (hx) { if (ix > N) { i2 = jmod1(ix, N); F2 = 4; } else if (ix < 1) { i2 = jmod1(ix, N); F2 = 2; } else i2 = ix; }
+      else { if (jy > N) { i2 = N - jmod1(jy, N) + 1; j2 = i1; F2 = 6; } else if (jy < 1) { i2 = N + jy; j2 = N - i1 + 1; F2 = 5; } else j2 = jy; }
+      break;
+    case 4:
+      if (hx) { if (ix > N) { i2 = jmod1(ix, N); F2 = 1; } else if (ix < 1) { i2 = jmod1(ix, N); F2 = 3; } else i2 = ix; }
+      else { if (jy > N) { i2 = N - i1 + 1; j2 = N - jmod1(jy, N) + 1; F2 = 6; } else if (jy < 1) { i2 = N - i1 + 1; j2 = 1 - jy; F2 = 5; } else j2 = jy; }
+      break;
+    case 5:
+      if (hx) { if (ix > N) { i2 = N - j1 + 1; j2 = jmod1(ix, N); F2 = 3; } else if (ix < 1) { i2 = j1; j2 = 1 - ix; F2 = 1; } else i2 = ix; }
+      else { if (jy > N) { j2 = jmod1(jy, N); F2 = 2; } else if (jy < 1) { i2 = N - i1 + 1; j2 = 1 - jy; F2 = 4; } else j2 = jy; }
+      break;
+    default:
+      if (hx) { if (ix > N) { i2 = j1; j2 = N - jmod1(ix, N) + 1; F2 = 3; } else if (ix < 1) { i2 = N - j1 + 1; j2 = N - ix; F2 = 1; } else i2 = ix; }
+      else { if (jy > N) { i2 = N - i1 + 1; j2 = N - jmod1(jy, N) + 1; F2 = 4; } else if (jy < 1) { j2 = jmod1(jy, N); F2 = 2; } else j2 = jy; }
+      break;
+  }
+}
+// cube_wrap (gen_geometry.jl:128-345): a diagonal shift is x first, then y, with the
+// reference's orientation of the second leg after leaving faces 5 and 6 sideways.
+__device__ __forceinline__ void cube_wrap(int i1, int j1, int F1, int x, int y, int N, int& i2, int& j2, int& F2) {
+  if (!(x != 0 && y != 0)) {
+    cube_wrap1(i1, j1, F1, x, y, N, i2, j2, F2);
+    return;
+  }
+  int ia, ja, Fa;
+  cube_wrap1(i1, j1, F1, x, 0, N, ia, ja, Fa);
+  if (F1 == 6 && i1 + x < 1) cube_wrap1(ia, ja, Fa, -y, 0, N, i2, j2, F2);
+  else if (F1 == 6 && i1 + x > N) cube_wrap1(ia, ja, Fa, y, 0, N, i2, j2, F2);
+  else if (F1 == 5 && i1 + x > N) cube_wrap1(ia, ja, Fa, -y, 0, N, i2, j2, F2);
+  else if (F1 == 5 && i1 + x < 1) cube_wrap1(ia, ja, Fa, y, 0, N, i2, j2, F2);
+  else cube_wrap1(ia, ja, Fa, 0, y, N, i2, j2, F2);
+}
+
+// Beam of a bundled ray on the skybox (calc_skybox_I_nu_beam!,
+// Wormhole.source/source_model.jl:291-389): base texel, Gaussian in texels
+// (distort_gauss of the bundle shape, main/my_special_functions.jl:39-55) and its
+// half-width h <= 10.
+struct SkyBeam {
+  const uint8_t* rgb;
+  int N, face, i, j, h;
+  double fs, sx, sy, c, s;
+};
+__device__ __forceinline__ bool skybox_beam(const DevScene& S, const DevSky& K, const double* R, const double* b,
+                                            SkyBeam& B) {
+  if (!(wh_r<double>(S, R[1]) > 50) || R[1] == 0) return false;
+  const int k = R[1] > 0 ? 0 : 1;
+  B.rgb = K.rgb[k];
+  B.N = K.N[k];
+  if (!skybox_face(S, R, B.N, B.face, B.i, B.j)) return false;
+  B.fs = R[4];
+  const double dp = b[10] * B.fs;
+  const double m1 = sqrt(b[5] * b[5] + b[6] * b[6]), m2 = sqrt(b[7] * b[7] + b[8] * b[8]);
+  const double sx = dp * (m1 + m2), sy = dp * (m1 - m2);
+  const double th = b[4] + atan2(b[5] * b[8] + b[6] * b[7], b[5] * b[7] - b[6] * b[8]) / 2;
+  const double ia = 1.0 / (atan(2.0 / (B.N + 1)) / 2 + NO_DIV_ZERO);
+  double st, ct;
+  sincos(th, &st, &ct);
+  const double Sxx = ia * ia * (sx * sx * (ct * ct) + sy * sy * (st * st));
+  const double Syy = ia * ia * (sx * sx * (st * st) + sy * sy * (ct * ct));
+  const double Sxy = ia * ia * (sx * sx - sy * sy) * sin(2 * th) / 2;
+  const double rt = sqrt((Sxx - Syy) * (Sxx - Syy) + 4 * (Sxy * Sxy));
+  B.sx = sqrt((Sxx + Syy + rt) / 2);
+  B.sy = sqrt((Sxx + Syy - rt) / 2);
+  sincos(atan2(2 * Sxy, Sxx - Syy) / 2, &B.s, &B.c);
+  const double hm = ceil(B.sx > B.sy ? B.sx : B.sy);
+  if (!(hm >= 0)) return false;  // no finite beam
+  B.h = hm < 10 ? (int)hm : 10;
+  return true;
+}
+// D2_Gauss (main/my_special_functions.jl:28-33) weight and weighted RGB of beam texel (x, y)
+__device__ __forceinline__ double skybox_beam_texel(const SkyBeam& B, int x, int y, double w[3]) {
+  int i2, j2, F2;
+  cube_wrap(B.i, B.j, B.face, x, y, B.N, i2, j2, F2);
+  skybox_rgb(B.rgb, B.N, F2, i2, j2, w);
+  const double isx = 1 / (2 * (B.sx * B.sx)), isy = 1 / (2 * (B.sy * B.sy));
+  const double a = B.c * B.c * isx + B.s * B.s * isy;
+  const double bb = -B.s * B.c * isx + B.s * B.c * isy;
+  const double cc = B.s * B.s * isx + B.c * B.c * isy;
+  return (1 / (2 * PI * B.sx * B.sy)) * exp(-(a * (x * x) + 2 * bb * x * y + cc * (y * y)));
 }
 // Normal_Gauss (main/my_special_functions.jl:20-22) and skybox_I_nu_calc's line mix
 __device__ __forceinline__ double normal_gauss(double x, double sg, double mu) {

@@ -343,6 +343,9 @@ struct Integrator {
   }
 
   __device__ __forceinline__ bool all_thick() const {
+    // with ray bundles the reference's opacity test also reads bundle slots that are 0
+    // (below -log(atol) > 0), so it never holds
+    if (A.O.bundles) return false;
     int t = 1;
 #pragma unroll
     for (int m = 0; m < CMAX; ++m)
@@ -764,10 +767,32 @@ struct Integrator {
 
   // calc_xyY sums (color_theory.jl:67-102) and the outputs of one ray.
   __device__ __forceinline__ void finish(long long ray, int attempted, int accepted) {
+    const long long pose = ray / A.npix;
+    double b11[11];
+    if (A.O.bundles) bundle_slots(A.cams[pose], ray - pose * A.npix, A.x_pix, b11);
     if constexpr (KIND == WORMHOLE) {  // skybox_handling! (LttM.jl:54)
       double R[8], fs, w[3];
       gather(Ro, R);
-      if (skybox_texel(S, A.sky, R, fs, w)) {
+      SkyBeam B;
+      if (A.O.bundles) {
+        if (skybox_beam(S, A.sky, R, b11, B)) {
+          double acc[CMAX];
+#pragma unroll
+          for (int m = 0; m < CMAX; ++m) acc[m] = 0;
+          for (int x = -B.h; x <= B.h; ++x)
+            for (int y = -B.h; y <= B.h; ++y) {
+              const double g = skybox_beam_texel(B, x, y, w);
+#pragma unroll
+              for (int m = 0; m < CMAX; ++m) {
+                if (!col_ok(m)) continue;
+                const double f = scol[lane + m * L].f * B.fs;
+                acc[m] += g * skybox_I(w, f) * exp(-sT[m]) / (f * f * f);
+              }
+            }
+#pragma unroll
+          for (int m = 0; m < CMAX; ++m) sI[m] += acc[m];
+        }
+      } else if (skybox_texel(S, A.sky, R, fs, w)) {
 #pragma unroll
         for (int m = 0; m < CMAX; ++m) {
           if (!col_ok(m)) continue;
@@ -808,6 +833,11 @@ struct Integrator {
         int c = lane + m * L;
         A.out_rays[ray + (long long)(8 + 2 * c) * T] = sI[m];
         A.out_rays[ray + (long long)(9 + 2 * c) * T] = sT[m];
+      }
+      if (A.O.bundles) {  // the 11 bundle slots after the colours, split over the lanes
+#pragma unroll
+        for (int k = 0; k < 11; ++k)
+          if (k % L == lane) A.out_rays[ray + (long long)(A.O.n + k) * T] = b11[k];
       }
     }
   }

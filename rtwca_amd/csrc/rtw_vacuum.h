@@ -187,14 +187,35 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   long long ray = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (ray >= N) return;
   double X = 0, Y = 0, Z = 0;
-  bool thick = A.O.ncol > 0;
+  bool thick = A.O.ncol > 0 && !A.O.bundles;
   const bool wormhole = A.S.kind == WORMHOLE;
   double fs = 0, w[3];
   bool sky = false;
+  const long long pose = ray / A.npix;
+  double b11[11];
+  if (A.O.bundles) bundle_slots(A.cams[pose], ray - pose * A.npix, A.x_pix, b11);
   if (wormhole) {  // skybox_handling! (LttM.jl:54)
     double R[8];
     for (int j = 0; j < 8; ++j) R[j] = P.geo[ray * 8 + j];
-    sky = skybox_texel(A.S, A.sky, R, fs, w);
+    SkyBeam B;
+    if (A.O.bundles && skybox_beam(A.S, A.sky, R, b11, B)) {
+      // beam: per colour, the Gaussian-weighted sum over the (2h+1)^2 texels, added to I
+      for (int c = 0; c < A.O.ncol; ++c) {
+        double2* sp = reinterpret_cast<double2*>(P.spec + (ray * A.O.ncol + c) * 2);
+        double2 v = *sp;
+        const double f = A.colors[c].f * B.fs;
+        double acc = 0;
+        for (int x = -B.h; x <= B.h; ++x)
+          for (int y = -B.h; y <= B.h; ++y) {
+            const double g = skybox_beam_texel(B, x, y, w);
+            acc += g * skybox_I(w, f) * exp(-v.y) / (f * f * f);
+          }
+        v.x += acc;
+        *sp = v;
+      }
+    } else if (!A.O.bundles) {
+      sky = skybox_texel(A.S, A.sky, R, fs, w);
+    }
   }
   for (int c = 0; c < A.O.ncol; ++c) {
     const DevColor& k = A.colors[c];
@@ -225,6 +246,8 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
       A.out_rays[ray + (8 + 2 * c) * N] = v.x;
       A.out_rays[ray + (9 + 2 * c) * N] = v.y;
     }
+    if (A.O.bundles)
+      for (int k = 0; k < 11; ++k) A.out_rays[ray + (long long)(A.O.n + k) * N] = b11[k];
   }
 }
 

@@ -116,7 +116,7 @@ class Context:
         H = np.tan(np.pi * c0.fov_vert / 180 / 2)
         ny = int(np.rint(c0.x_pix * H / W))
         npix = c0.x_pix * ny * ncam
-        n = 8 + 2 * len(lam)
+        n = 8 + 2 * len(lam) + (11 if integ.ray_bundles else 0)
         xyz = np.zeros((npix, 3))
         cls = np.zeros(npix, dtype=np.uint8) if want_class else None
         steps = np.zeros((npix, 2), dtype=np.int32) if want_steps else None
@@ -195,12 +195,10 @@ def gen_image(scene, camera_pos=(0, 0, 0, 0), camera_dir=(0.0, 0.0), speed=0.0, 
               abs_tol=None, rel_tol=None, precision=abi.PREC_FP64, schedule=abi.SCHED_PERSISTENT):
     """gen_image with the reference's keywords (LttM.jl:104-107).  `tolerance` fills both
     tolerances as the reference does (LttM.jl:131-132); abs_tol/rel_tol override them."""
-    if ray_bundles:
-        raise RTWError("ray bundles (skybox anti-aliasing) are not provided by rtwca_amd")
     cam = abi.make_camera(camera_pos, camera_dir, speed, camera_point, x_pix, fov_hor, fov_vert)
     integ = abi.make_integrator(tolerance if rel_tol is None else rel_tol,
                                 tolerance if abs_tol is None else abs_tol, max_dt_scale, max_steps,
-                                precision, schedule)
+                                precision, schedule, ray_bundles=ray_bundles)
     ctx = context()
     r = ctx.render(scene, [cam], integ, list(colors), want_rays=returnrays)
     if print_num_pix:

@@ -117,6 +117,26 @@ def make_wormhole_fixture():
     print("wrote", wormhole_fixture_name(), "steps", int(r["steps"][:, 0].sum()))
 
 
+def make_bundle_fixtures():
+    """Ray bundles: the wormhole with beam-sampled skyboxes, and the opaque sphere of the
+    first workload, where bundles switch the opacity stop off (the reference's
+    calc_terminate also reads zero bundle slots)."""
+    sc, cam, it, lam, (s1, s2) = wormhole_workload(*WORMHOLE_FIXTURE)
+    it.ray_bundles = 1
+    O.load_skybox(1, s1)
+    O.load_skybox(2, s2)
+    r = O.render(sc, cam, it, lam, nthreads=os.cpu_count() or 1, want_rays=True)
+    name = os.path.join(HERE, "oracle_wormhole_%dpx_sky%d_bundles.npz" % WORMHOLE_FIXTURE)
+    np.savez_compressed(name, xyz=r["xyz"], cls=r["cls"], steps=r["steps"], rays=r["rays"], lam=np.array(lam))
+    print("wrote", name, "steps", int(r["steps"][:, 0].sum()))
+    w = scenes.config(1, x_pix=12, n_lambda=16)
+    w.integ.ray_bundles = 1
+    r = O.render(w.scene, w.cameras[0], w.integ, w.lam, nthreads=os.cpu_count() or 1, want_rays=True)
+    name = os.path.join(HERE, "oracle_cfg1_12px_16l_bundles.npz")
+    np.savez_compressed(name, xyz=r["xyz"], cls=r["cls"], steps=r["steps"], rays=r["rays"], lam=np.array(w.lam))
+    print("wrote", name, "steps", int(r["steps"][:, 0].sum()), "classes", np.bincount(r["cls"], minlength=5))
+
+
 def make_flops():
     out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
                          "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
@@ -153,10 +173,12 @@ if __name__ == "__main__":
         sys.exit(0)
     elif "--wormhole-only" in sys.argv:
         make_wormhole_fixture()
+        make_bundle_fixtures()
         sys.exit(0)
     elif "--flops-only" not in sys.argv:
         make_fixtures()
         make_synch_fixture()
         make_m87_fixture()
         make_wormhole_fixture()
+        make_bundle_fixtures()
     make_flops()

@@ -135,3 +135,117 @@ def test_uniform_sky_end_to_end():
                             + w[2] * ng(f, 9e-5, 0.002141)) / f ** 3
             assert ray[8 + 2 * c] == pytest.approx(want, rel=1e-12)
     assert sides == {True, False}
+
+
+# ---------------------------------------------------------------- ray bundles
+def _face_dir(face, i, j, N):
+    """Direction of the centre of texel (i, j) of a face: the inverse of calc_skybox_coord."""
+    u, v = 2 * (i - 0.5) / N - 1, 2 * (j - 0.5) / N - 1
+    d = {1: (1, -u, v), 2: (-u, -1, v), 3: (-1, u, v), 4: (u, 1, v), 5: (-u, -v, -1), 6: (-u, v, 1)}[face]
+    d = np.array(d, float)
+    return d / np.linalg.norm(d)
+
+
+def test_cube_face_inverse_projection():
+    N = 8
+    for face in range(1, 7):
+        for i in range(1, N + 1):
+            for j in range(1, N + 1):
+                d = _face_dir(face, i, j, N)
+                got = O.skybox_coord(_ray_dir(d), N)
+                assert got[4] == face and got[0] == i and got[1] == j
+
+
+def test_cube_wrap_follows_the_cube():
+    """Shifts by one texel across every face edge land on the geometrically adjacent texel
+    (the centres are one texel apart), and shifts inside a face move (i, j)."""
+    N = 8
+    step = np.arctan(2 / N)  # largest centre-to-centre angle of neighbouring texels
+    for face in range(1, 7):
+        assert O.cube_wrap(4, 5, face, 0, 0, N) == (4, 5, face)
+        assert O.cube_wrap(4, 5, face, 2, 0, N) == (6, 5, face)
+        assert O.cube_wrap(4, 5, face, 0, -3, N) == (4, 2, face)
+        for i in range(1, N + 1):
+            for j in range(1, N + 1):
+                if 1 < i < N and 1 < j < N:
+                    continue
+                for x, y in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                    i2, j2, f2 = O.cube_wrap(i, j, face, x, y, N)
+                    assert 1 <= i2 <= N and 1 <= j2 <= N and 1 <= f2 <= 6
+                    ang = math.acos(min(1.0, _face_dir(face, i, j, N) @ _face_dir(f2, i2, j2, N)))
+                    assert ang <= 1.01 * step, (face, i, j, x, y, (i2, j2, f2), ang)
+
+
+def test_distort_gauss_is_the_covariance_transform():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        sx, sy, th = rng.uniform(0.1, 3), rng.uniform(0.1, 3), rng.uniform(-PI, PI)
+        a, b = rng.uniform(0.5, 5), rng.uniform(0.5, 5)
+        Rm = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+        S = np.diag([a, b]) @ Rm @ np.diag([sx * sx, sy * sy]) @ Rm.T @ np.diag([a, b])
+        sxn, syn, thn = O.distort_gauss(sx, sy, th, a, b)
+        Rn = np.array([[math.cos(thn), -math.sin(thn)], [math.sin(thn), math.cos(thn)]])
+        assert np.allclose(Rn @ np.diag([sxn * sxn, syn * syn]) @ Rn.T, S, rtol=1e-10, atol=1e-12 * S.max())
+    # D2_Gauss is that normal density: it integrates to 1
+    sxn, syn, thn = O.distort_gauss(1.2, 0.7, 0.4, 1.5, 2.0)
+    st = min(sxn, syn) / 4
+    ext = 8 * max(sxn, syn)
+    xs = np.arange(-ext, ext + st / 2, st)
+    tot = sum(O.d2_gauss(x, y, sxn, syn, thn) for x in xs for y in xs) * st * st
+    assert tot == pytest.approx(1.0, rel=1e-6)
+
+
+def test_bundle_params_of_the_camera():
+    sc, pos, d, p, beta = scenes.initialize_world("wormhole")
+    cam = abi.make_camera(pos, d, beta, p, 9, 85, 60)
+    Mc = O.camera_matrix(sc, cam)
+    r = _dneg(pos[1])[0]
+    B3 = Mc[1:, 1:] / np.array([1.0, 1 / r, 1 / (r * math.sin(pos[2]) + 1e-24)])[:, None]
+    W, H = math.tan(PI * 85 / 360), math.tan(PI * 60 / 360)
+    ny = O.camera_y_pix(cam)
+    rh, rv = 2 * W / 10, 2 * H / (ny + 1)
+    for i, j in ((0, 0), (4, 3), (8, ny - 1), (2, 5)):
+        w, h = rh * (i + 0.5) - W, rv * (j + 0.5) - H
+        th = math.atan(math.hypot(w, h))
+        ph = math.atan2(h, w)
+        n = np.array([-math.sin(th) * math.cos(ph), -math.sin(th) * math.sin(ph), -math.cos(th)])
+        E = 1 / (abs(n[2]) + 1e-24)
+        J = np.cross(n, [0, 0, -1.0])
+        if J @ J < 1e-3:
+            J = np.array([0, -1.0, 0])
+        J, nr = B3 @ J, B3 @ n
+        J, nr = J / np.linalg.norm(J), nr / np.linalg.norm(nr)
+        a = np.array([nr[1], 1 - nr[1] ** 2 / (1 - nr[0]), -nr[1] * nr[2] / (1 - nr[0])])
+        b = np.array([nr[2], -nr[1] * nr[2] / (1 - nr[0]), 1 - nr[2] ** 2 / (1 - nr[0])])
+        assert abs(a @ nr) < 1e-12 and abs(b @ nr) < 1e-12  # the bundle plane is transverse to the ray
+        a, b = a / (a @ a), b / (b @ b)  # the reference's squared-norm division
+        want = [0, 0, 0, 0, 0, (a @ J) * (1 + E) / 2, (b @ J) * (1 + E) / 2, (a @ J) * (1 - E) / 2,
+                (b @ J) * (1 - E) / 2, 0, abs(math.atan(rh) / 2 * n[2])]
+        assert np.allclose(O.bundle_params(cam, i, j), want, rtol=1e-10, atol=1e-14)
+
+
+def test_beam_skybox_of_a_uniform_sky_is_the_weight_sum():
+    sc, pos, d, p, beta = scenes.initialize_world("wormhole")
+    cam = abi.make_camera(pos, d, beta, p, 10, 85, 60)
+    lam = [450.0, 630.0]
+    O.load_skybox(1, np.full((24, 32, 3), 180, dtype=np.uint8))
+    O.load_skybox(2, np.full((24, 32, 3), 90, dtype=np.uint8))
+    single = O.render(sc, cam, abi.make_integrator(1e-4, 1e-4, 0.1, 1e4), lam, nthreads=4, want_rays=True)
+    beam = O.render(sc, cam, abi.make_integrator(1e-4, 1e-4, 0.1, 1e4, ray_bundles=True), lam, nthreads=4,
+                    want_rays=True)
+    assert beam["rays"].shape[1] == single["rays"].shape[1] + 11
+    assert np.array_equal(beam["steps"], single["steps"])  # bundle slots are not integrated
+    ratio = beam["rays"][:, 8] / single["rays"][:, 8]
+    assert np.all(ratio > 0.5) and np.all(ratio < 1.0 + 1e-12)
+    # the ratio is the sum of the D2_Gauss weights over the (2h+1)^2 texels
+    k = 7
+    ray, b = beam["rays"][k], beam["rays"][k][-11:]
+    N = 8
+    dp = b[10] * ray[4]
+    m1, m2 = math.hypot(b[5], b[6]), math.hypot(b[7], b[8])
+    mu = b[4] + math.atan2(b[5] * b[8] + b[6] * b[7], b[5] * b[7] - b[6] * b[8]) / 2
+    sbs = math.atan(2 / (N + 1)) / 2
+    sx, sy, th = O.distort_gauss(dp * (m1 + m2), dp * (m1 - m2), mu, 1 / (sbs + 1e-24), 1 / (sbs + 1e-24))
+    h = min(math.ceil(max(sx, sy)), 10)
+    wsum = sum(O.d2_gauss(x, y, sx, sy, th) for x in range(-h, h + 1) for y in range(-h, h + 1))
+    assert ratio[k] == pytest.approx(wsum, rel=1e-12)
