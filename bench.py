@@ -45,7 +45,7 @@ PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32 (AMD published)
 PEAK_FP64_VALU = 78.6   # TFLOP/s, MI355X vector FP64
 TRAFFIC_SUMMARY = {3: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_1024_mixed_hbm.md"),
                    5: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_synch_64x512_mixed_hbm.md")}
-WORKLOADS = {"kerr": 3, "synch": 5, "reference-suite": 0}
+WORKLOADS = {"kerr": 3, "synch": 5, "reference-suite": 0, "skybox": 7}
 
 # The reference's own timed tests (main/initialization.jl:352-512: 500 px wide, FOV 85 x
 # 60, 350:30:740 nm, max_dt_scale 0.1, max_steps 1e4) and the times it recorded
@@ -165,7 +165,7 @@ def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
     cams = (abi.Camera * len(w.cameras))(*w.cameras)
     integ = abi.make_integrator(w.integ.rtol, w.integ.atol, w.integ.max_dt_scale, w.integ.max_steps,
                                 precision, w.integ.schedule if schedule is None else schedule,
-                                w.integ.fp64_radius)
+                                w.integ.fp64_radius, ray_bundles=bool(w.integ.ray_bundles))
     lam = np.ascontiguousarray(w.lam, dtype=np.float64)
     c0 = w.cameras[0]
     ny = int(np.rint(c0.x_pix * np.tan(np.pi * c0.fov_vert / 360) / np.tan(np.pi * c0.fov_hor / 360)))
@@ -267,6 +267,34 @@ def reference_suite(a):
                       "place of its 4096-wide sky PNGs"}))
 
 
+def skybox_bench(a, torch):
+    """The wormhole with ray bundles (its skybox sampled over each ray's beam) at x_pix wide
+    (default 2048), FOV 85 x 60, 350:30:740 nm, tolerance 1e-4, on synthetic 1024-texel
+    skies; mixed precision, wavefront schedule, timed with HIP events through
+    rtw_render_dev.  The texture reads happen in finish_kernel."""
+    from rtwca_amd import abi, host, scenes, skybox
+    ctx = host.Context(0)
+    s1, s2 = skybox.default_skyboxes(1024)
+    ctx.load_skybox(1, s1)
+    ctx.load_skybox(2, s2)
+    sc, pos, d, p, beta = scenes.initialize_world("wormhole")
+    cam = abi.make_camera(pos, d, beta, p, a.x_pix or 2048, 85, 60)
+    w = scenes.Workload("wormhole_bundles_skybox_%d" % cam.x_pix, sc, [cam],
+                        abi.make_integrator(1e-4, 1e-4, 0.1, 1e4, abi.PREC_MIXED, abi.SCHED_WAVEFRONT, 3.0,
+                                            ray_bundles=True),
+                        [float(x) for x in range(350, 751, 30)])
+    times, xyz, cls, stp = gpu_render(ctx, w, abi.PREC_MIXED, a.steps, a.warmup, torch)
+    st = ctx.last_stats()
+    ms = float(np.mean(times))
+    npix = xyz.shape[0]
+    print(json.dumps({"metric": "rays/s, wormhole with ray bundles and beam-sampled skyboxes", "value": npix / (ms * 1e-3),
+                      "unit": "rays/s", "ms_per_step": ms, "steps": a.steps, "warmup": a.warmup,
+                      "config": {"workload": w.name, "rays": npix, "n_lambda": len(w.lam), "sky_face": 1024,
+                                 "sky_bytes": 2 * s1.nbytes},
+                      "rk45_steps_per_s": int(stp[:, 0].to(torch.int64).sum().item()) / (ms * 1e-3),
+                      "kernels_ms": {"lane_group": st["group_ms"], "vacuum": st["vacuum_ms"]}}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -284,6 +312,9 @@ def main():
         return
     if a.workload == "reference-suite":
         reference_suite(a)
+        return
+    if a.workload == "skybox":
+        skybox_bench(a, torch)
         return
     idx = WORKLOADS[a.workload]
     x_pix = a.x_pix or (1024 if idx == 3 else 512)

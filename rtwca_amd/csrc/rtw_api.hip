@@ -969,6 +969,7 @@ int rtw_render_adjoint(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* cam
   if (scene->kind != RTW_SCENE_KERR_THICK_FF)
     return fail(c, RTW_E_UNSUPPORTED, "the adjoint covers the thick free-free disk scene");
   if (scene->flags & RTW_SCENE_REF_F32) return fail(c, RTW_E_UNSUPPORTED, "the adjoint runs in FP64 only");
+  if (opts->ray_bundles) return fail(c, RTW_E_UNSUPPORTED, "the adjoint has no ray bundles");
   if (n_lambda > 64) return fail(c, RTW_E_UNSUPPORTED, "at most 64 wavelengths");
   if (!(opts->rtol > 0) || !(opts->atol > 0) || !(opts->max_dt_scale > 0) || !(opts->max_steps >= 1))
     return fail(c, RTW_E_ARG, "invalid tolerances/max_dt_scale/max_steps");

@@ -343,9 +343,6 @@ struct Integrator {
   }
 
   __device__ __forceinline__ bool all_thick() const {
-    // with ray bundles the reference's opacity test also reads bundle slots that are 0
-    // (below -log(atol) > 0), so it never holds
-    if (A.O.bundles) return false;
     int t = 1;
 #pragma unroll
     for (int m = 0; m < CMAX; ++m)
@@ -358,7 +355,9 @@ struct Integrator {
     if (step_count >= max_steps || radial<KIND>(S, r) > S.r_escape || (KIND != MINK && r <= S.r_horizon) ||
         phi > 8 * PI)
       return true;
-    return all_thick();
+    // with ray bundles the reference's opacity test also reads bundle slots that are 0
+    // (below -log(atol) > 0), so it never stops a ray
+    return !A.O.bundles && all_thick();
   }
   __device__ __forceinline__ bool terminate(double max_steps, double step_count) const {
     return stop_test(rr, ph, max_steps, step_count);

@@ -187,44 +187,39 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   long long ray = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (ray >= N) return;
   double X = 0, Y = 0, Z = 0;
-  bool thick = A.O.ncol > 0 && !A.O.bundles;
+  bool thick = A.O.ncol > 0;  // the class of the final state (bundles only change when rays stop)
   const bool wormhole = A.S.kind == WORMHOLE;
   double fs = 0, w[3];
-  bool sky = false;
+  bool sky = false, beam = false;
+  SkyBeam B;
   const long long pose = ray / A.npix;
   double b11[11];
   if (A.O.bundles) bundle_slots(A.cams[pose], ray - pose * A.npix, A.x_pix, b11);
   if (wormhole) {  // skybox_handling! (LttM.jl:54)
     double R[8];
     for (int j = 0; j < 8; ++j) R[j] = P.geo[ray * 8 + j];
-    SkyBeam B;
-    if (A.O.bundles && skybox_beam(A.S, A.sky, R, b11, B)) {
-      // beam: per colour, the Gaussian-weighted sum over the (2h+1)^2 texels, added to I
-      for (int c = 0; c < A.O.ncol; ++c) {
-        double2* sp = reinterpret_cast<double2*>(P.spec + (ray * A.O.ncol + c) * 2);
-        double2 v = *sp;
-        const double f = A.colors[c].f * B.fs;
-        double acc = 0;
-        for (int x = -B.h; x <= B.h; ++x)
-          for (int y = -B.h; y <= B.h; ++y) {
-            const double g = skybox_beam_texel(B, x, y, w);
-            acc += g * skybox_I(w, f) * exp(-v.y) / (f * f * f);
-          }
-        v.x += acc;
-        *sp = v;
-      }
-    } else if (!A.O.bundles) {
-      sky = skybox_texel(A.S, A.sky, R, fs, w);
-    }
+    if (A.O.bundles) beam = skybox_beam(A.S, A.sky, R, b11, B);
+    else sky = skybox_texel(A.S, A.sky, R, fs, w);
   }
+  // one pass over the colours: (I, tau) read once, the skybox added, written back only
+  // when the final rays are requested
   for (int c = 0; c < A.O.ncol; ++c) {
     const DevColor& k = A.colors[c];
     double2 v = *reinterpret_cast<const double2*>(P.spec + (ray * A.O.ncol + c) * 2);
-    if (sky) {
+    if (beam) {  // the Gaussian-weighted sum over the beam's (2h+1)^2 texels
+      const double f = k.f * B.fs;
+      double acc = 0;
+      for (int x = -B.h; x <= B.h; ++x)
+        for (int y = -B.h; y <= B.h; ++y) {
+          const double g = skybox_beam_texel(B, x, y, w);
+          acc += g * skybox_I(w, f) * exp(-v.y) / (f * f * f);
+        }
+      v.x += acc;
+    } else if (sky) {
       const double f = k.f * fs;
       v.x += skybox_I(w, f) * exp(-v.y) / (f * f * f);
-      if (A.out_rays) *reinterpret_cast<double2*>(P.spec + (ray * A.O.ncol + c) * 2) = v;
     }
+    if ((beam || sky) && A.out_rays) *reinterpret_cast<double2*>(P.spec + (ray * A.O.ncol + c) * 2) = v;
     double I = v.x * k.cx;
     X += I * k.wx; Y += I * k.wy; Z += I * k.wz;
     if (!(v.y >= A.O.tau_thr)) thick = false;
