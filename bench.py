@@ -53,6 +53,7 @@ WORKLOADS = {"kerr": 3, "synch": 5, "reference-suite": 0, "skybox": 7}
 # "LttM" = Float64 on the CPU with all threads).
 REFERENCE_SUITE = [
     # name, initialize_world key, tolerance, 5P seconds, LttM seconds
+    ("cart_sph", "minkowski_t,x,y,z_sphere", 1e-4, 1.900872014, 2.52330864),
     ("sph_sph", "minkowski_t,r,θ,ϕ_sphere", 1e-4, 3.048285446, 46.675094802),
     ("bh_simple", "schwarzschild_t,r,θ,ϕ_simpledisk", 1e-4, 3.360003761, 47.78255016),
     ("kerr", "kerr_boyer_lindquist", 1e-5, 4.244008561, 68.373113457),
@@ -60,6 +61,10 @@ REFERENCE_SUITE = [
     ("wormhole", "wormhole", 1e-4, 23.663417359, None),  # ray_bundles=true (initialization.jl:436-439)
 ]
 BUNDLED = {"wormhole"}
+# skybox face sizes of the tests that read skyboxes: the reference's Cubemap_Sky_22-512x512
+# (Minkowski_Sphere_Cartesian.source/source_model.jl:12) and 4096-wide sky PNGs
+# (Wormhole.source/source_model.jl:12-16), replaced by synthetic cube maps of those sizes
+SKY_FACE = {"cart_sph": 512, "wormhole": 1024}
 
 
 def parse():
@@ -233,12 +238,13 @@ def reference_suite(a):
     import time
     from rtwca_amd import abi, host, scenes, skybox
     ctx = host.Context(0)
-    s1, s2 = skybox.default_skyboxes(1024)  # the reference's 4096-wide skies, synthetic
-    ctx.load_skybox(1, s1)
-    ctx.load_skybox(2, s2)
     lam = [float(x) for x in range(350, 751, 30)]
     rows = []
     for name, key, tol, t5p, tcpu in REFERENCE_SUITE:
+        if name in SKY_FACE:
+            s1, s2 = skybox.default_skyboxes(SKY_FACE[name])
+            ctx.load_skybox(1, s1)
+            ctx.load_skybox(2, s2)
         sc, pos, d, p, beta = scenes.initialize_world(key)
         cam = abi.make_camera(pos, d, beta, p, a.x_pix or 500, 85, 60)
         row = {"test": name, "rays": None, "reference_5P_s": t5p, "reference_LttM_s": tcpu}
@@ -264,7 +270,7 @@ def reference_suite(a):
         rows.append(row)
     print(json.dumps({"reference_suite": rows, "note": "reference times are TestResults.csv on unstated hardware; "
                       "the wormhole test runs with ray bundles as there, on synthetic 1024-texel skyboxes in "
-                      "place of its 4096-wide sky PNGs"}))
+                      "place of its 4096-wide sky PNGs; cart_sph on a synthetic 512-texel skybox 1"}))
 
 
 def skybox_bench(a, torch):

@@ -51,8 +51,30 @@ enum rtw_scene_kind {
   RTW_SCENE_KERR_THICK_FF = 4,      /* Kerr + Gaussian thick disk, free-free with van Hoof Gaunt */
   RTW_SCENE_KERR_SYNCH = 5,         /* Kerr + thermal-synchrotron torus and conical jet */
   RTW_SCENE_POWEHI_M87 = 6,         /* main/Powehi.source: the M87* model */
-  RTW_SCENE_WORMHOLE = 7            /* main/Wormhole.source: Interstellar's Dneg wormhole */
+  RTW_SCENE_WORMHOLE = 7,           /* main/Wormhole.source: Interstellar's Dneg wormhole */
+  RTW_SCENE_MINKOWSKI_CART_SPHERE = 8,  /* main/Minkowski_Sphere_Cartesian.source */
+  RTW_SCENE_SYNCHRO_TEST = 9        /* main/Synchro_Test.source: flat synchrotron cylinder */
 };
+
+/* RTW_SCENE_MINKOWSKI_CART_SPHERE follows main/Minkowski_Sphere_Cartesian.source:
+ * flat space in (t, x, y, z) (source_geometry.jl:8-82: diagonal metric, no
+ * Christoffel symbols, identity vierbein, no singularity), a sphere of radius r_out
+ * (the reference: 1) about the origin radiating as a blackbody at `temperature`
+ * with a_nu = absorb (source_model.jl:51-89), max_dt = max_dt_scale (1 + |x| + |y|
+ * + |z|) (:130-132), rays stopping beyond r_escape from the origin (:158-172).
+ * Every finished ray, whatever its end, adds skybox 1 in the direction of its
+ * spatial k (source_model.jl:191-198, 406-431; the texel, or with ray bundles the
+ * Gaussian beam); skybox 1 must be loaded with rtw_load_skybox.  The camera's
+ * pos is (t, x, y, z). */
+
+/* RTW_SCENE_SYNCHRO_TEST follows main/Synchro_Test.source: flat space in (t, x, y,
+ * z) as above, a static cylinder sqrt(x^2 + y^2) <= r_out (the reference: 1)
+ * (source_model.jl:216-223) in a uniform field B^z = bfield Gauss (:50-56) with the
+ * power-law synchrotron of RTW_SCENE_POWEHI_M87 (p = 2.5, gamma 30 .. 1e9,
+ * anisotropy; :152-213), without that scene's floors on b^2 and sin theta;
+ * max_dt = max_dt_scale (1 + |x| + |y|) (:229-231); no skybox.  Where rounding
+ * makes cos^2 theta exceed 1 the reference raises a DomainError (sqrt of a
+ * negative number); the library gives NaN for that ray. */
 
 /* RTW_SCENE_WORMHOLE follows main/Wormhole.source: coordinates (t, l, theta, phi) with
  * l the proper radial distance (negative on the far side), throat radius 1, half-length

@@ -139,7 +139,13 @@ enum SceneKind {
   KERR_SYNCH = 5,         // new: Kerr geometry + thermal-synchrotron torus and conical jet
   POWEHI_M87 = 6,         // Powehi.source: the M87* model (wedge disk + power-law synchrotron jet)
   WORMHOLE = 7,           // Wormhole.source: Interstellar's Dneg wormhole, skyboxes on both sides
+  MINKOWSKI_CART_SPHERE = 8,  // Minkowski_Sphere_Cartesian.source: the sphere in (t, x, y, z), one skybox
+  SYNCHRO_TEST = 9,           // Synchro_Test.source: power-law synchrotron cylinder in flat (t, x, y, z)
 };
+// The Kerr family shares the metric, Christoffel symbols and max_dt rule of Kerr.source.
+inline bool kerr_family(int k) { return k >= KERR_DISK && k <= POWEHI_M87; }
+// Flat space in Cartesian coordinates (t, x, y, z): no horizon, no coordinate singularity.
+inline bool cartesian(int k) { return k == MINKOWSKI_CART_SPHERE || k == SYNCHRO_TEST; }
 enum SceneFlags {
   REF_F32 = 1,  // Kerr family: Float32 Christoffel, horizon/ISCO and max_dt_scale as the reference
   GAUNT_FP64 = 2,  // thick disk: smooth Gaunt interpolation (double, plain bilinear weights)
@@ -230,7 +236,7 @@ inline void scene_init(Scene& s) {
   s.r_s = 2 * s.mass;
   if (s.kind == SCHWARZSCHILD_DISK) {
     s.r_ev = s.r_s;  // Schwarzschild_...source_model.jl:358 terminates at (1+10 rtol) r_s
-  } else if (s.kind >= KERR_DISK && s.kind != WORMHOLE) {
+  } else if (kerr_family(s.kind)) {
     if (s.flags & REF_F32) {  // Kerr.source/source_geometry.jl:7-17 in Float32
       float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
       s.r_ev = (double)(rs / 2 + std::sqrt((rs * rs / 4) - a * a));
@@ -250,7 +256,12 @@ inline void scene_init(Scene& s) {
 // ---------------------------------------------------------------------------
 // calc_lower_metric! : Minkowski source_geometry.jl:10-16, Schwarzschild :15-21, Kerr :55-75.
 template <class R> void lower_metric(const Scene& s, const R* x, R g[4][4]) {
-  if (s.kind == MINKOWSKI_SPHERE) {
+  if (cartesian(s.kind)) {  // Minkowski_Sphere_Cartesian source_geometry.jl:8-13, Synchro_Test :8-16
+    g[0][0] = R(-1.0);
+    g[1][1] = R(1.0);
+    g[2][2] = R(1.0);
+    g[3][3] = R(1.0);
+  } else if (s.kind == MINKOWSKI_SPHERE) {
     g[0][0] = R(-1.0);
     g[1][1] = R(1.0);
     g[2][2] = sq(x[1]);
@@ -332,7 +343,9 @@ template <class T> void kerr_christoffel_T(T r, T th, T a, T rs, T G[4][4][4]) {
 // Kerr :195-243.  G must be zero-initialised once per ray (LttM.jl:21); only the
 // listed entries are written.
 template <class R> void christoffel(const Scene& s, const R* x, R G[4][4][4]) {
-  if (s.kind == MINKOWSKI_SPHERE) {
+  if (cartesian(s.kind)) {  // all zero (Minkowski_Sphere_Cartesian source_geometry.jl:47-59)
+    return;
+  } else if (s.kind == MINKOWSKI_SPHERE) {
     R d(NO_DIV_ZERO);
     G[1][2][2] = -x[1];
     G[1][3][3] = -x[1] * sq(sin(x[2]));
@@ -390,6 +403,7 @@ template <class R> std::pair<bool, R> near_singularity(const Scene& s, const R* 
   R th_new = ray[2] + dth;
   R dr = h * ray[5];
   R new_r = ray[1] + dr;
+  if (cartesian(s.kind)) return {false, h};  // Minkowski_Sphere_Cartesian source_geometry.jl:64-66
   if (s.kind == MINKOWSKI_SPHERE) {
     if (fabs(th_new) <= R(10) * R(atol3) / fabs(new_r + d))
       return {true, R(2) * fabs(ray[2]) / (fabs(ray[6] * R(37)) + d)};
@@ -416,6 +430,7 @@ template <class R> R julia_mod(R x, R y) {
 // keepinbounds!: Minkowski source_geometry.jl:138-163 (with the r <= 0 reflection),
 // Schwarzschild :160-186, Kerr :275-301.
 template <class R> void keepinbounds(const Scene& s, R* ray) {
+  if (cartesian(s.kind)) return;  // Minkowski_Sphere_Cartesian source_geometry.jl:72-74
   R pi(PI), twopi(2 * PI);
   if (s.kind == MINKOWSKI_SPHERE && ray[1] <= R(0)) {
     ray[1] = -ray[1] + R(NO_DIV_ZERO);
@@ -441,7 +456,12 @@ template <class R> R pad_max_dt(const Scene& s, const R* ray, double max_dt_scal
   const double tenth_f = (double)0.1f, half_f = 0.5, tiny_f = (double)1e-12f;
   R d(NO_DIV_ZERO), pi(PI);
   R eps;
-  bool kerr = s.kind >= KERR_DISK && s.kind != WORMHOLE;
+  // Minkowski_Sphere_Cartesian source_model.jl:130-132 (all three axes), Synchro_Test
+  // source_model.jl:229-231 (x and y only): the scale grows with the distance
+  if (s.kind == MINKOWSKI_CART_SPHERE)
+    return R(max_dt_scale) * (R(1) + fabs(ray[1]) + fabs(ray[2]) + fabs(ray[3]));
+  if (s.kind == SYNCHRO_TEST) return R(max_dt_scale) * (R(1) + fabs(ray[1]) + fabs(ray[2]));
+  bool kerr = kerr_family(s.kind);
   double mds = (kerr && (s.flags & REF_F32)) ? (double)(float)max_dt_scale : max_dt_scale;
   if (ray[2] < R(PI / 8) || ray[2] > R(7 * PI / 8)) {
     eps = R(std::min(mds, tenth_f));
@@ -640,6 +660,10 @@ template <class R> int pw_region(const Scene& s, const R* x) {
 template <class R> bool is_fire(const Scene& s, const R* x) {
   switch (s.kind) {
     case MINKOWSKI_SPHERE: return x[1] <= R(s.r_out);
+    // Minkowski_Sphere_Cartesian source_model.jl:82-89: the unit sphere about the origin
+    case MINKOWSKI_CART_SPHERE: return sqrt(sq(x[1]) + sq(x[2]) + sq(x[3])) <= R(s.r_out);
+    // Synchro_Test source_model.jl:216-223: the unit cylinder about the z axis
+    case SYNCHRO_TEST: return sqrt(sq(x[1]) + sq(x[2])) <= R(s.r_out);
     case SCHWARZSCHILD_DISK:
     case KERR_DISK:
       return R(s.r_in) <= x[1] && x[1] <= R(s.r_out) && R(s.th_lo) <= x[2] && x[2] <= R(s.th_hi);
@@ -691,7 +715,9 @@ template <class R> bool synch_velocity(const Scene& s, const R* x, int reg, R u[
 // get_source_velocity!: Minkowski source_model.jl:288-294, Schwarzschild :312-333,
 // Kerr :109-137 (prograde Keplerian in matter, FIDO outside).
 template <class R> void source_velocity(const Scene& s, const R* x, const R g[4][4], R u[4]) {
-  if (s.kind == MINKOWSKI_SPHERE || s.kind == WORMHOLE) {  // static observer (Wormhole source_model.jl:111-118)
+  // static observer (Wormhole source_model.jl:111-118, Minkowski_Sphere_Cartesian :138-144,
+  // Synchro_Test :237-243)
+  if (s.kind == MINKOWSKI_SPHERE || s.kind == WORMHOLE || cartesian(s.kind)) {
     u[0] = R(1); u[1] = R(0); u[2] = R(0); u[3] = R(0);
     return;
   }
@@ -765,7 +791,11 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
       fp[1] = R(0);
     }
   }
-  if (s.kind == POWEHI_M87) {  // Powehi.source/source_model.jl:154-175
+  // Powehi.source/source_model.jl:154-175; Synchro_Test source_model.jl:152-171 is the
+  // same with B = (0, 0, 10 G) in flat space and neither the 1e-12 floor on "b^2" nor the
+  // clamp of cos^2 to 1
+  if (s.kind == POWEHI_M87 || s.kind == SYNCHRO_TEST) {
+    const bool pw = s.kind == POWEHI_M87;
     fp[0] = R(2.5);   // power-law index p
     fp[1] = R(30.0);  // gamma_min
     fp[2] = R(1e9);   // gamma_max
@@ -791,7 +821,7 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
     for (int i = 0; i < 4; ++i) bm += g[i][1] * sq(b2);
     for (int i = 0; i < 4; ++i) bm += g[i][2] * sq(b3);
     for (int i = 0; i < 4; ++i) bm += g[i][3] * sq(b4);
-    bm = rmax(R((double)1e-12f), bm);
+    if (pw) bm = rmax(R((double)1e-12f), bm);
     fp[3] = R((double)PW_NE0) * bm;
     // calc_b_angle (:118-149)
     R bk(0.0);
@@ -807,7 +837,7 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
     ps = R(0.0);
     for (int i = 0; i < 4; ++i) ps += g[i][3] * x[i + 4];
     bk += b4 * ps;
-    fp[4] = rmin(sq(bk) / (bm * sq(fs)), R(1.0)) + R(NO_DIV_ZERO);
+    fp[4] = pw ? rmin(sq(bk) / (bm * sq(fs)), R(1.0)) + R(NO_DIV_ZERO) : sq(bk) / (bm * sq(fs));
     fp[5] = R((double)9.34e-11f) * sqrt(bm);  // cyclotron frequency in nm^-1
     fp[6] = R(s.temperature);                 // disk blackbody temperature
   }
@@ -841,14 +871,17 @@ template <class R> void radiative_coefficients(const Scene& s, const R* x, const
     }
     return;
   }
-  if (s.kind == POWEHI_M87) {  // Powehi.source/source_model.jl:186-223
-    const int zone = pw_region(s, x);
+  // Powehi.source/source_model.jl:186-223; Synchro_Test source_model.jl:182-213 (the
+  // cylinder is the jet, sin theta without the 1e-12 floor)
+  if (s.kind == POWEHI_M87 || s.kind == SYNCHRO_TEST) {
+    const bool pw = s.kind == POWEHI_M87;
+    const int zone = pw ? pw_region(s, x) : (is_fire(s, x) ? 2 : 3);
     if (zone == 1) {
       a_nu = R((double)(float)s.absorb);
       j_nu = a_nu * planck(fp[6], nu);
     } else if (zone == 2) {
       const R p = fp[0], gmin = fp[1], gmax = fp[2], c2 = fp[4];
-      const R st = sqrt(rmax(R((double)1e-12f), R(1) - c2));
+      const R st = pw ? sqrt(rmax(R((double)1e-12f), R(1) - c2)) : sqrt(R(1) - c2);
       j_nu = R((double)(PW_NE0 * pw_synch_em())) * (fp[5] / R((double)9.34e-11f));
       a_nu = R((double)(PW_NE0 * pw_synch_ab())) / nu;
       const R gd = pow(gmin, R(1) - p) - pow(gmax, R(1) - p);
@@ -974,10 +1007,16 @@ template <class R> struct RaySys {
   void keep(R* ray) const { keepinbounds(*sc, ray); }
   // calc_terminate: Minkowski source_model.jl:308-322, Schwarzschild :356-370, Kerr :168-182.
   bool terminate(const R* ray, double max_steps, double step_count) const {
-    bool horizon = (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE) && (ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev));
-    // the wormhole tests r(l) (Wormhole.source/source_model.jl:136)
-    const R rr = (sc->kind == WORMHOLE) ? wh_r(ray[1]) : ray[1];
-    if (step_count >= max_steps || rr > R(sc->r_escape) || horizon || ray[3] > R(8 * PI)) return true;
+    if (cartesian(sc->kind)) {
+      // Minkowski_Sphere_Cartesian source_model.jl:158-172, Synchro_Test :257-271: the
+      // Euclidean distance from the origin; no horizon and no winding test
+      if (step_count >= max_steps || sqrt(sq(ray[1]) + sq(ray[2]) + sq(ray[3])) > R(sc->r_escape)) return true;
+    } else {
+      bool horizon = (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE) && (ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev));
+      // the wormhole tests r(l) (Wormhole.source/source_model.jl:136)
+      const R rr = (sc->kind == WORMHOLE) ? wh_r(ray[1]) : ray[1];
+      if (step_count >= max_steps || rr > R(sc->r_escape) || horizon || ray[3] > R(8 * PI)) return true;
+    }
     R thr(-std::log(o.atol));
     for (int i = 9; i < n; i += 2)
       if (ray[i] < thr) return false;
@@ -989,9 +1028,13 @@ template <class R> struct RaySys {
   }
   int classify(const R* ray) const {
     if (!isfinite(ray[1])) return CLS_BOUNDARY;
-    if (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE && ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev))
-      return CLS_HORIZON;
-    if (sc->kind != WORMHOLE && ray[1] > R(sc->r_escape)) return CLS_ESCAPE;
+    if (cartesian(sc->kind)) {
+      if (sqrt(sq(ray[1]) + sq(ray[2]) + sq(ray[3])) > R(sc->r_escape)) return CLS_ESCAPE;
+    } else {
+      if (sc->kind != MINKOWSKI_SPHERE && sc->kind != WORMHOLE && ray[1] <= R((1 + 10 * o.rtol) * sc->r_ev))
+        return CLS_HORIZON;
+      if (sc->kind != WORMHOLE && ray[1] > R(sc->r_escape)) return CLS_ESCAPE;
+    }
     if (sc->kind == WORMHOLE && wh_r(ray[1]) > R(sc->r_escape)) return CLS_ESCAPE;
     R thr(-std::log(o.atol));
     bool thick = true;
@@ -1141,16 +1184,11 @@ inline double skybox_I_nu(const Skybox& sb, int i, int j, double f) {
                   B * normal_gauss(f, 9e-5, 0.002141));
 }
 
-// calc_skybox_coord (:175-276): cube face and in-face pixel (1-based) of the direction
-// R_sph_to_cart e k (vierbein diag(1, 1, r(l), r(l) sin theta)).
-inline void skybox_coord(const double* ray, int N, int& i, int& j, double& u, double& v, int& face) {
-  const double rl = wh_r(ray[1]);
-  const double st = std::sin(ray[2]), ct = std::cos(ray[2]), sp = std::sin(ray[3]), cp = std::cos(ray[3]);
-  const double e2 = rl, e3 = rl * st;  // vierbein diagonal (t, l entries are 1)
-  // (R e) k with R = R_sph_to_cart (gen_geometry.jl:110-123)
-  const double vx = (st * cp) * ray[5] + (ct * cp * e2) * ray[6] + (-sp * e3) * ray[7];
-  const double vy = (st * sp) * ray[5] + (ct * sp * e2) * ray[6] + (cp * e3) * ray[7];
-  const double vz = ct * ray[5] + (-st * e2) * ray[6];
+// calc_skybox_coord, face and in-face part (Wormhole.source/source_model.jl:186-276,
+// Minkowski_Sphere_Cartesian source_model.jl:200-297): cube face and in-face pixel
+// (1-based) of the direction (vx, vy, vz).
+inline void skybox_face_coord(double vx, double vy, double vz, int N, int& i, int& j, double& u, double& v,
+                              int& face) {
   const double nrm = std::sqrt(vx * vx + vy * vy + vz * vz);
   const double x = vx / nrm, y = vy / nrm, z = vz / nrm;
   const double ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
@@ -1169,6 +1207,36 @@ inline void skybox_coord(const double* ray, int N, int& i, int& j, double& u, do
   j = std::min((int)std::floor(N * (v + 1) / 2) + 1, N);
 }
 
+// calc_skybox_coord of the wormhole (Wormhole.source/source_model.jl:175-185): the
+// direction is R_sph_to_cart e k (vierbein diag(1, 1, r(l), r(l) sin theta)).
+inline void skybox_coord(const double* ray, int N, int& i, int& j, double& u, double& v, int& face) {
+  const double rl = wh_r(ray[1]);
+  const double st = std::sin(ray[2]), ct = std::cos(ray[2]), sp = std::sin(ray[3]), cp = std::cos(ray[3]);
+  const double e2 = rl, e3 = rl * st;  // vierbein diagonal (t, l entries are 1)
+  // (R e) k with R = R_sph_to_cart (gen_geometry.jl:110-123)
+  const double vx = (st * cp) * ray[5] + (ct * cp * e2) * ray[6] + (-sp * e3) * ray[7];
+  const double vy = (st * sp) * ray[5] + (ct * sp * e2) * ray[6] + (cp * e3) * ray[7];
+  const double vz = ct * ray[5] + (-st * e2) * ray[6];
+  skybox_face_coord(vx, vy, vz, N, i, j, u, v, face);
+}
+
+// The skybox a finished ray samples, and its face coordinates: the wormhole's rays that
+// ended at r(l) > 50 see skybox 1 (l > 0) or 2 (l < 0) (Wormhole.source/source_model.jl:
+// 393-443); the Cartesian sphere adds skybox 1 to every ray, in the direction of its
+// spatial k (Minkowski_Sphere_Cartesian source_model.jl:191-198, 406-431).  0: no sky.
+inline int skybox_of_ray(const Scene& s, const double* ray, int& i, int& j, double& u, double& v, int& face) {
+  int which = 0;
+  if (s.kind == WORMHOLE) {
+    if (!(wh_r(ray[1]) > 50) || ray[1] == 0) return 0;
+    which = ray[1] > 0 ? 1 : 2;
+    skybox_coord(ray, skybox(which).N, i, j, u, v, face);
+  } else if (s.kind == MINKOWSKI_CART_SPHERE) {
+    which = 1;
+    skybox_face_coord(ray[5], ray[6], ray[7], skybox(1).N, i, j, u, v, face);
+  }
+  return which;
+}
+
 // cubemap_coord_calc (gen_geometry.jl:347-370): face-local -> absolute cross coordinates
 inline void cubemap_abs(int i, int j, int face, int N, int& ia, int& ja) {
   static const int di[7] = {0, 0, 1, 2, 3, 1, 1}, dj[7] = {0, 1, 1, 1, 1, 0, 2};
@@ -1176,17 +1244,16 @@ inline void cubemap_abs(int i, int j, int face, int N, int& ia, int& ja) {
   ja = j + dj[face] * N;
 }
 
-// skybox_handling! without ray bundles (:393-443): rays that ended at r(l) > 50 take the
-// intensity of their skybox texel, attenuated by exp(-tau), per colour.
+// skybox_handling! without ray bundles (Wormhole.source/source_model.jl:393-443,
+// Minkowski_Sphere_Cartesian source_model.jl:406-431): a ray that sees a skybox
+// (skybox_of_ray) takes the intensity of its texel, attenuated by exp(-tau), per colour.
 inline void skybox_handling(const Scene& s, double* ray, int ncol, const double* freq) {
-  if (s.kind != WORMHOLE) return;
-  const double rl = wh_r(ray[1]);
-  if (!(rl > 50) || ray[1] == 0) return;
-  const Skybox& sb = skybox(ray[1] > 0 ? 1 : 2);
-  const double fs = ray[4];  // -[1,0,0,0]^T g k with g_tt = -1
   int i, j, face, ia, ja;
   double u, v;
-  skybox_coord(ray, sb.N, i, j, u, v, face);
+  const int which = skybox_of_ray(s, ray, i, j, u, v, face);
+  if (which == 0) return;
+  const Skybox& sb = skybox(which);
+  const double fs = ray[4];  // -[1,0,0,0]^T g k with g_tt = -1
   cubemap_abs(i, j, face, sb.N, ia, ja);
   for (int c = 0; c < ncol; ++c) {
     const double f = freq[c] * fs;
@@ -1337,7 +1404,9 @@ inline void boost_z(double beta, M4 L) {
 // calc_inv_vierbein: Minkowski source_geometry.jl:45-50, Schwarzschild :53-60, Kerr :112-121.
 inline void inv_vierbein(const Scene& s, const double* x, M4 e) {
   m4_eye(e);
-  if (s.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:83-90
+  if (cartesian(s.kind)) {  // identity (Minkowski_Sphere_Cartesian source_geometry.jl:38-41)
+    return;
+  } else if (s.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:83-90
     e[2][2] = 1 / wh_r(x[1]);
     e[3][3] = 1 / (wh_r(x[1]) * std::sin(x[2]) + NO_DIV_ZERO);
   } else if (s.kind == MINKOWSKI_SPHERE) {
@@ -1606,15 +1675,13 @@ inline double d2_gauss(double x, double y, double sx, double sy, double th) {
 // skybox_handling! with ray bundles (Wormhole.source/source_model.jl:291-389, 393-443):
 // a Gaussian-weighted (2h+1)^2 texel neighbourhood, the beam shape from the bundle slots.
 inline void skybox_handling_beam(const Scene& s, double* ray, int ncol, const double* freq, const double* b11) {
-  if (s.kind != WORMHOLE) return;
-  const double rl = wh_r(ray[1]);
-  if (!(rl > 50) || ray[1] == 0) return;
-  const Skybox& sb = skybox(ray[1] > 0 ? 1 : 2);
-  const int N = sb.N;
-  const double fs = ray[4];
   int i, j, face;
   double u, v;
-  skybox_coord(ray, N, i, j, u, v, face);
+  const int which = skybox_of_ray(s, ray, i, j, u, v, face);
+  if (which == 0) return;
+  const Skybox& sb = skybox(which);
+  const int N = sb.N;
+  const double fs = ray[4];
   const double dp = b11[10] * fs;
   const double m1 = std::sqrt(sq(b11[5]) + sq(b11[6])), m2 = std::sqrt(sq(b11[7]) + sq(b11[8]));
   const double dmaj = dp * (m1 + m2), dmin = dp * (m1 - m2);
@@ -1641,7 +1708,9 @@ inline void skybox_handling_beam(const Scene& s, double* ray, int ncol, const do
 }
 
 // is_singularity (Kerr source_geometry.jl:307-315 etc.): camera validity guard.
-inline bool is_singularity(const double* x) {
+// The Cartesian scenes have none (Minkowski_Sphere_Cartesian source_geometry.jl:80-82).
+inline bool is_singularity(const Scene& s, const double* x) {
+  if (cartesian(s.kind)) return false;
   return std::fabs(x[1]) <= (double)1e-32f || std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - PI) <= 1e-32;
 }
 

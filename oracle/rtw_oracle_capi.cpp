@@ -316,6 +316,7 @@ int rtwo_render(const rtw_scene* s, const rtw_camera* c, const rtw_integrator* o
                 uint8_t* out_cls, int64_t* out_steps, double* out_rays, double* out_seconds) {
   if (!gaunt_table().loaded && s->kind == KERR_THICK_FF) return -10;
   if (s->kind == WORMHOLE && (skybox(1).N == 0 || skybox(2).N == 0)) return -11;
+  if (s->kind == MINKOWSKI_CART_SPHERE && skybox(1).N == 0) return -11;
   Scene sc = to_scene(s);
   Camera cam = to_cam(c);
   Opts op = to_opts(o);

@@ -21,6 +21,8 @@ SCENE_KERR_THICK_FF = 4
 SCENE_KERR_SYNCH = 5
 SCENE_POWEHI_M87 = 6
 SCENE_WORMHOLE = 7
+SCENE_MINKOWSKI_CART_SPHERE = 8
+SCENE_SYNCHRO_TEST = 9
 
 SCENE_REF_F32 = 1
 SCENE_GAUNT_FP64 = 2

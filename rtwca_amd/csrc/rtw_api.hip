@@ -158,7 +158,9 @@ static void boost(double b, M4 L) {
 static void inv_vierbein(const DevScene& S, const double* x, M4 e) {
   mzero(e);
   for (int i = 0; i < 4; ++i) e[i][i] = 1;
-  if (S.kind == MINK) {
+  if (S.kind == MINK_CART || S.kind == SYNCH_TEST) {
+    return;  // identity (Minkowski_Sphere_Cartesian source_geometry.jl:38-41)
+  } else if (S.kind == MINK) {
     e[2][2] = 1 / (x[1] + NO_DIV_ZERO);
     e[3][3] = 1 / (x[1] * std::sin(x[2]) + NO_DIV_ZERO);
   } else if (S.kind == WORMHOLE) {  // Wormhole.source/source_geometry.jl:83-90
@@ -189,8 +191,10 @@ static void inv_vierbein(const DevScene& S, const double* x, M4 e) {
 }
 
 // is_singularity (Kerr source_geometry.jl:307-315 and the others; the wormhole's
-// Wormhole.source/source_geometry.jl:198-205 tests theta only, l = 0 is the throat)
+// Wormhole.source/source_geometry.jl:198-205 tests theta only, l = 0 is the throat; the
+// Cartesian scenes have none, Minkowski_Sphere_Cartesian source_geometry.jl:80-82)
 static bool cam_is_singular(const double* x, int kind = 0) {
+  if (kind == RTW_SCENE_MINKOWSKI_CART_SPHERE || kind == RTW_SCENE_SYNCHRO_TEST) return false;
   const bool pole = std::fabs(x[2]) <= (double)1e-32f || std::fabs(x[2] - M_PI) <= 1e-32;
   if (kind == RTW_SCENE_WORMHOLE) return pole;
   return std::fabs(x[1]) <= (double)1e-32f || pole;
@@ -341,8 +345,10 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.inv_2h2 = 1.0 / (2 * s.h_over_r * s.h_over_r);
   d.r_escape = s.r_escape;
   d.r_photon15 = 1.5 * (3 * s.mass);
-  d.ref_f32 = (s.kind >= RTW_SCENE_KERR_DISK && (s.flags & RTW_SCENE_REF_F32)) ? 1 : 0;
-  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_WORMHOLE)
+  const bool kerr_family = s.kind >= RTW_SCENE_KERR_DISK && s.kind <= RTW_SCENE_POWEHI_M87;
+  const bool cartesian = s.kind == RTW_SCENE_MINKOWSKI_CART_SPHERE || s.kind == RTW_SCENE_SYNCHRO_TEST;
+  d.ref_f32 = (kerr_family && (s.flags & RTW_SCENE_REF_F32)) ? 1 : 0;
+  if (s.kind < RTW_SCENE_MINKOWSKI_SPHERE || s.kind > RTW_SCENE_SYNCHRO_TEST)
     return fail(ctx, RTW_E_UNSUPPORTED, "scene kind not supported");
   if (s.kind == RTW_SCENE_SCHWARZSCHILD_DISK) {
     d.r_ev = d.rs;
@@ -356,7 +362,7 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
     d.r_ev = 0;
     if (!(ctx->sky_n[0] > 0 && ctx->sky_n[1] > 0))
       return fail(ctx, RTW_E_ARG, "wormhole scene needs both skyboxes (rtw_load_skybox)");
-  } else if (s.kind >= RTW_SCENE_KERR_DISK) {
+  } else if (kerr_family) {
     if (d.ref_f32) {
       float M = (float)s.mass, a = (float)s.spin, rs = 2.0f * M;
       d.r_ev = (double)(rs / 2 + std::sqrt(rs * rs / 4 - a * a));
@@ -369,6 +375,12 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   }
   d.r_horizon = (s.kind == RTW_SCENE_MINKOWSKI_SPHERE) ? 0.0 : (1 + 10 * o.rtol) * d.r_ev;
   if (s.kind == RTW_SCENE_WORMHOLE) d.r_horizon = -HUGE_VAL;  // no horizon; l < 0 is the far side
+  if (cartesian) {
+    d.r_ev = 0;
+    d.r_horizon = -HUGE_VAL;  // flat space: no horizon (the stop tests use the distance)
+  }
+  if (s.kind == RTW_SCENE_MINKOWSKI_CART_SPHERE && !(ctx->sky_n[0] > 0))
+    return fail(ctx, RTW_E_ARG, "the Cartesian sphere scene needs skybox 1 (rtw_load_skybox)");
   // pad_max_dt epsilons: Kerr uses Float32(max_dt_scale) (Kerr.source/source_model.jl:93-95)
   double mds = (d.ref_f32) ? (double)(float)o.max_dt_scale : o.max_dt_scale;
   const double tenth_f = (double)0.1f;
@@ -376,9 +388,10 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.eps_pole = mds < tenth_f ? mds : tenth_f;
   d.eps_photon = o.max_dt_scale / 10;
   d.bb_a = s.absorb;
-  if (s.kind == RTW_SCENE_POWEHI_M87) {
+  if (s.kind == RTW_SCENE_POWEHI_M87 || s.kind == RTW_SCENE_SYNCHRO_TEST) {
     // Powehi.source/source_model.jl:6-30: Float32 constants of the file; p = 2.5,
-    // gamma_min = 30, gamma_max = 1e9 (calc_fluid_params!, :157-162)
+    // gamma_min = 30, gamma_max = 1e9 (calc_fluid_params!, :157-162).  Synchro_Test
+    // source_model.jl:1-36, 152-171 holds the same constants and power law.
     const float map_scale = 9.62e12f;
     const float synch_em = (float)(7.694e-37 * (double)map_scale * 2.8e7);
     const float synch_ab = (float)(2.8173e-24 * (double)map_scale);
@@ -394,10 +407,12 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
     d.pw_ej = -(p - 1) / 2;
     d.pw_ea = -(p + 2) / 2;
     d.pw_ephi = -p / 2;
-    d.pw_omj = (double)(0.5f * (float)d.a / ((float)d.rs * (float)d.r_ev));
-    d.pw_noz = (double)((10.0f - 1.2f) / (75.0f - (float)s.r_jet));
-    d.pw_rjet = (double)(float)s.r_jet;
-    d.pw_bphi = (double)(float)s.bfield;
+    if (s.kind == RTW_SCENE_POWEHI_M87) {
+      d.pw_omj = (double)(0.5f * (float)d.a / ((float)d.rs * (float)d.r_ev));
+      d.pw_noz = (double)((10.0f - 1.2f) / (75.0f - (float)s.r_jet));
+      d.pw_rjet = (double)(float)s.r_jet;
+    }
+    d.pw_bphi = (double)(float)s.bfield;  // B^phi (M87*), B^z (Synchro_Test)
     d.bb_a = (double)(float)s.absorb;
   }
   if (s.kind == RTW_SCENE_KERR_SYNCH) {
@@ -468,7 +483,7 @@ static std::vector<DevColor> make_colors(const DevScene& S, const double* lam, i
     c.lc = std::log10(c.uf);
     double r = FF_NU0 / c.f;
     c.if3 = r * r * r;
-    if (S.kind == POWEHI) c.lc = std::log2(c.f);
+    if (S.kind == POWEHI || S.kind == SYNCH_TEST) c.lc = std::log2(c.f);
     if (S.kind == KERR_SYNCH) {
       // x3_c = (nu / nu_s)^(1/3) at unit shift (f in nm^-1 -> 1e7 cm^-1) and
       // K_c = log2 Q - x3_c log2(e) - 8 log2 x3_c

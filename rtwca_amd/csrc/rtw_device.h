@@ -27,7 +27,13 @@ constexpr double GAM2_FACTOR = 2.1798723611035e-18 / K_B;         // Ry / k_B
 constexpr double FF_T0 = 1e4;           // thick disk opacity normalisation temperature
 constexpr double FF_NU0 = 1.0 / 550.0;  // and frequency (nm^-1)
 
-enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5, POWEHI = 6, WORMHOLE = 7 };
+enum Kind { MINK = 1, SCHW = 2, KERR = 3, KERR_FF = 4, KERR_SYNCH = 5, POWEHI = 6, WORMHOLE = 7, MINK_CART = 8,
+            SYNCH_TEST = 9 };
+// Flat space in Cartesian coordinates (t, x, y, z): Minkowski_Sphere_Cartesian.source and
+// Synchro_Test.source.
+template <int KIND> constexpr bool cart_v = KIND == MINK_CART || KIND == SYNCH_TEST;
+// Power-law synchrotron of Powehi.source (its jet; the whole cylinder of Synchro_Test).
+template <int KIND> constexpr bool plaw_v = KIND == POWEHI || KIND == SYNCH_TEST;
 constexpr int SCENE_GAUNT_FP64 = 2;  // Gaunt interpolation in FP64 (scene flag)
 enum { CLS_HORIZON = 1, CLS_ESCAPE = 2, CLS_OPAQUE = 3, CLS_BOUNDARY = 4 };
 
@@ -308,8 +314,27 @@ template <class T> __device__ __forceinline__ T wh_drdl(const DevScene& S, T l) 
   const T sg = l > T(0) ? T(1) : (l < T(0) ? T(-1) : T(0));
   return T(2) * sg * (atan(x) + (x - T(1)) / (T(1) + x * x)) / T(PI);
 }
-// The radial coordinate the termination tests compare with r_escape: r(l) for the
-// wormhole (Wormhole.source/source_model.jl:136), r otherwise.
+// (a a + b b) + c c rounded as written (no contraction), as the reference evaluates
+// sqrt(x^2 + y^2 + z^2).
+__device__ __forceinline__ double sumsq3(double a, double b, double c) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+}
+__device__ __forceinline__ float sumsq3(float a, float b, float c) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
+}
+// The coordinates of the termination test (calc_terminate): r and phi; in the Cartesian
+// scenes the distance from the origin and no winding test (Minkowski_Sphere_Cartesian
+// source_model.jl:158-172, Synchro_Test :257-271).
+template <int KIND> __device__ __forceinline__ double pos_r(const double* x) {
+  if constexpr (cart_v<KIND>) return sqrt(sumsq3(x[1], x[2], x[3]));
+  else return x[1];
+}
+template <int KIND> __device__ __forceinline__ double pos_phi(const double* x) {
+  if constexpr (cart_v<KIND>) return 0.0;
+  else return x[3];
+}
+// The radial coordinate the termination tests compare with r_escape, from pos_r: r(l)
+// for the wormhole (Wormhole.source/source_model.jl:136), r otherwise.
 template <int KIND> __device__ __forceinline__ double radial(const DevScene& S, double r) {
   if constexpr (KIND == WORMHOLE) return wh_r<double>(S, r);
   else return r;
@@ -324,7 +349,34 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
   const T kt = x[4], kr = x[5], kth = x[6], kph = x[7];
   o.d[0] = kt; o.d[1] = kr; o.d[2] = kth; o.d[3] = kph;
   o.w = T(0);
-  if constexpr (KIND == MINK) {
+  if constexpr (cart_v<KIND>) {
+    // flat (t, x, y, z): no Christoffel symbols (Minkowski_Sphere_Cartesian
+    // source_geometry.jl:47-59); static source, shift = k^t (source_model.jl:138-144)
+    o.d[4] = T(0); o.d[5] = T(0); o.d[6] = T(0); o.d[7] = T(0);
+    o.sh = kt;
+    if constexpr (KIND == MINK_CART) {
+      // the sphere about the origin (source_model.jl:82-89)
+      o.matter = sqrt(sumsq3(x[1], x[2], x[3])) <= T(S.r_out);
+    } else {
+      // Synchro_Test: the cylinder about the z axis (source_model.jl:216-223) in B = (0, 0,
+      // B^z) (:50-56).  For the static source u = (1, 0, 0, 0) transform_B_b (:62-113)
+      // gives b = (0, 0, 0, B^z) and "b^2" = B^z^2, calc_b_angle (:116-149) b.k = B^z k^z:
+      // cos^2 = (B^z k^z)^2 / (B^z^2 shift^2), with neither floor nor clamp
+      constexpr bool F = fast_math_v<T>;
+      o.matter = sqrt(sumsq3(x[1], x[2], T(0))) <= T(S.r_out);
+      o.zone = o.matter ? 2 : 3;
+      if (o.matter) {
+        const T B4 = T(S.pw_bphi);
+        const T bm = B4 * B4, bk = B4 * x[7];
+        const T c2 = bk * bk / (bm * (kt * kt));
+        const T sb = sqrt_<F>(bm), st = sqrt_<F>(T(1) - c2);
+        const T phi = exp_(T(S.pw_ephi) * log_(T(1) - T(0.99f) * c2));
+        o.w = T(S.pw_cj) * sb * st * phi;
+        o.w2 = T(S.pw_ca) * phi;
+        o.w3 = rcp_<F>(T(9.34e-11f) * sb * st);
+      }
+    }
+  } else if constexpr (KIND == MINK) {
     // Minkowski_Sphere_Spherical.source/source_geometry.jl:93-108
     T s, c;
     sc_(th, s, c);
@@ -553,13 +605,16 @@ __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T 
 // pad_max_dt (Minkowski source_model.jl:266-282, Schwarzschild :286-306, Kerr :86-103).
 template <int KIND>
 __device__ __forceinline__ double pad_max_dt(const DevScene& S, const double* x) {
+  // Minkowski_Sphere_Cartesian source_model.jl:130-132, Synchro_Test :229-231
+  if constexpr (KIND == MINK_CART) return S.eps_main * (1 + fabs(x[1]) + fabs(x[2]) + fabs(x[3]));
+  if constexpr (KIND == SYNCH_TEST) return S.eps_main * (1 + fabs(x[1]) + fabs(x[2]));
   const double d = NO_DIV_ZERO;
   double eps;
   if (x[2] < PI / 8 || x[2] > 7 * PI / 8) eps = S.eps_pole;
   else if (KIND == SCHW && x[1] <= S.r_photon15) eps = S.eps_photon;
   else eps = S.eps_main;
   if (KIND == WORMHOLE && fabs(x[1]) < 100 * S.wh_a) eps = eps / 10;  // Wormhole source_model.jl:100-102
-  double A = (KIND >= KERR && KIND != WORMHOLE) ? fabs(1.0 / (eps / fabs(x[5])) + d)
+  double A = (KIND >= KERR && KIND <= POWEHI) ? fabs(1.0 / (eps / fabs(x[5])) + d)
                                                 : fabs(1.0 / (eps / (fabs(x[5]) + d)) + d);
   double th = x[2];
   double mth = (PI - th < th) ? PI - th : th;
@@ -577,6 +632,7 @@ __device__ __forceinline__ double pad_max_dt(const DevScene& S, const double* x)
 // Minkowski, the origin).
 template <int KIND>
 __device__ __forceinline__ bool near_singularity(const double* x, double h, double atol, double& ts) {
+  if constexpr (cart_v<KIND>) return false;  // Minkowski_Sphere_Cartesian source_geometry.jl:64-66
   const double d = NO_DIV_ZERO;
   double th_new = x[2] + h * x[6];
   double new_r = x[1] + h * x[5];
@@ -605,6 +661,7 @@ __device__ __forceinline__ double jmod(double x, double y) {
 
 // keepinbounds! (Minkowski source_geometry.jl:138-163; Kerr :275-301).
 template <int KIND> __device__ __forceinline__ void keepinbounds(double* x) {
+  if constexpr (cart_v<KIND>) return;  // Minkowski_Sphere_Cartesian source_geometry.jl:72-74
   if (KIND == MINK && x[1] <= 0) {
     x[1] = -x[1] + NO_DIV_ZERO;
     x[5] = -x[5];
@@ -627,6 +684,7 @@ template <int KIND> __device__ __forceinline__ void keepinbounds(double* x) {
 // keepinbounds! that also reports the parity of its theta reflections (each one
 // negates theta and k^theta; the phi shifts by pi have unit derivative).
 template <int KIND> __device__ __forceinline__ int keepinbounds_parity(double* x) {
+  if constexpr (cart_v<KIND>) return 0;
   int flips = 0;
   if (KIND == MINK && x[1] <= 0) {
     x[1] = -x[1] + NO_DIV_ZERO;
@@ -742,26 +800,39 @@ __device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, i
   return ok ? g : 1.2f;
 }
 
-// Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
-// Skybox lookup (Wormhole.source/source_model.jl:150-276, 393-443; gen_geometry.jl:
-// 110-123, 347-370): a ray that ended at r(l) > 50 takes the texel of skybox 1 (l > 0)
-// or 2 (l < 0) in the direction R_sph_to_cart e k.  Returns false when none applies;
-// otherwise the texel's weighted RGB (1, 0.843, 0.914 x value/255) and the shift k^t.
+// Skybox lookup (Wormhole.source/source_model.jl:150-276, 393-443; Minkowski_Sphere_
+// Cartesian source_model.jl:175-297, 406-431; gen_geometry.jl:110-123, 347-370): a
+// wormhole ray that ended at r(l) > 50 takes the texel of skybox 1 (l > 0) or 2 (l < 0)
+// in the direction R_sph_to_cart e k; every ray of the Cartesian sphere takes skybox 1
+// in the direction of its spatial k.  Returns false when none applies; otherwise the
+// texel's weighted RGB (1, 0.843, 0.914 x value/255) and the shift k^t.
 struct DevSky {
   const uint8_t* rgb[2];  // [3N][4N][3] each, row-major
   int N[2];
 };
-// Face and in-face texel (1-based) of the direction R_sph_to_cart e k
-// (calc_skybox_coord); false for a non-finite direction.
+// The skybox (0: skybox 1, 1: skybox 2) a finished ray takes, or -1.
+__device__ __forceinline__ int sky_of_ray(const DevScene& S, const double* R) {
+  if (S.kind == MINK_CART) return 0;
+  if (S.kind != WORMHOLE || !(wh_r<double>(S, R[1]) > 50) || R[1] == 0) return -1;
+  return R[1] > 0 ? 0 : 1;
+}
+// Face and in-face texel (1-based) of the ray's direction (calc_skybox_coord): R_sph_to_cart
+// e k for the wormhole, (k^x, k^y, k^z) for the Cartesian sphere; false for a non-finite
+// direction.
 __device__ __forceinline__ bool skybox_face(const DevScene& S, const double* R, int N, int& face, int& i, int& j) {
-  const double rl = wh_r<double>(S, R[1]);
-  double st, ct, sp, cp;
-  sincos(R[2], &st, &ct);
-  sincos(R[3], &sp, &cp);
-  const double e2 = rl, e3 = rl * st;
-  const double vx = (st * cp) * R[5] + (ct * cp * e2) * R[6] + (-sp * e3) * R[7];
-  const double vy = (st * sp) * R[5] + (ct * sp * e2) * R[6] + (cp * e3) * R[7];
-  const double vz = ct * R[5] + (-st * e2) * R[6];
+  double vx, vy, vz;
+  if (S.kind == MINK_CART) {
+    vx = R[5]; vy = R[6]; vz = R[7];
+  } else {
+    const double rl = wh_r<double>(S, R[1]);
+    double st, ct, sp, cp;
+    sincos(R[2], &st, &ct);
+    sincos(R[3], &sp, &cp);
+    const double e2 = rl, e3 = rl * st;
+    vx = (st * cp) * R[5] + (ct * cp * e2) * R[6] + (-sp * e3) * R[7];
+    vy = (st * sp) * R[5] + (ct * sp * e2) * R[6] + (cp * e3) * R[7];
+    vz = ct * R[5] + (-st * e2) * R[6];
+  }
   const double nrm = sqrt(vx * vx + vy * vy + vz * vz);
   const double x = vx / nrm, y = vy / nrm, z = vz / nrm;
   const double ax = fabs(x), ay = fabs(y), az = fabs(z);
@@ -802,8 +873,8 @@ __device__ __forceinline__ void skybox_rgb(const uint8_t* rgb, int N, int face, 
 }
 __device__ __forceinline__ bool skybox_texel(const DevScene& S, const DevSky& K, const double* R, double& fs,
                                              double w[3]) {
-  if (!(wh_r<double>(S, R[1]) > 50) || R[1] == 0) return false;
-  const int b = R[1] > 0 ? 0 : 1;
+  const int b = sky_of_ray(S, R);
+  if (b < 0) return false;
   int face, i, j;
   if (!skybox_face(S, R, K.N[b], face, i, j)) return false;
   skybox_rgb(K.rgb[b], K.N[b], face, i, j, w);
@@ -877,8 +948,8 @@ struct SkyBeam {
 };
 __device__ __forceinline__ bool skybox_beam(const DevScene& S, const DevSky& K, const double* R, const double* b,
                                             SkyBeam& B) {
-  if (!(wh_r<double>(S, R[1]) > 50) || R[1] == 0) return false;
-  const int k = R[1] > 0 ? 0 : 1;
+  const int k = sky_of_ray(S, R);
+  if (k < 0) return false;
   B.rgb = K.rgb[k];
   B.N = K.N[k];
   if (!skybox_face(S, R, B.N, B.face, B.i, B.j)) return false;
@@ -922,6 +993,7 @@ __device__ __forceinline__ double skybox_I(const double w[3], double f) {
                   w[2] * normal_gauss(f, 9e-5, 0.002141));
 }
 
+// Per-stage scalars of the transfer equation, shared by the L lanes of a ray.
 struct StageSc {
   double sh;   // shift
   double P;    // blackbody: -a_nu/shift; free-free: -C rho^2 / shift^4; synchrotron: -Q n / shift;

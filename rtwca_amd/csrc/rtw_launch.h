@@ -29,6 +29,8 @@ RTW_DECLARE_KIND(launch_kind_kerr_ff)
 RTW_DECLARE_KIND(launch_kind_kerr_synch)
 RTW_DECLARE_KIND(launch_kind_powehi)
 RTW_DECLARE_KIND(launch_kind_wormhole)
+RTW_DECLARE_KIND(launch_kind_mink_cart)
+RTW_DECLARE_KIND(launch_kind_synch_test)
 
 template <class K> hipError_t persistent_grid(K kern, size_t lds, long long need, int num_cu, long long& grid, int& bpc) {
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, BLOCK, lds);
@@ -114,6 +116,8 @@ inline hipError_t launch_pass(int which, const KArgs& A, const KArgs* dA, const 
     case KERR_SYNCH: return launch_kind_kerr_synch(which, A, dA, Q, mixed, num_cu, st, li);
     case POWEHI: return launch_kind_powehi(which, A, dA, Q, mixed, num_cu, st, li);
     case WORMHOLE: return launch_kind_wormhole(which, A, dA, Q, mixed, num_cu, st, li);
+    case MINK_CART: return launch_kind_mink_cart(which, A, dA, Q, mixed, num_cu, st, li);
+    case SYNCH_TEST: return launch_kind_synch_test(which, A, dA, Q, mixed, num_cu, st, li);
     default: li.msg = "unsupported scene"; return hipErrorInvalidValue;
   }
 }

@@ -158,7 +158,7 @@ __device__ __forceinline__ StageSc make_stage(const DevScene& S, const StageOut<
       // and shift^(1/3)
       ss.P = -(S.sy_Q * (double)o.w) / sh;
       ss.lg = cbrt(sh);
-    } else if constexpr (KIND == POWEHI) {
+    } else if constexpr (plaw_v<KIND>) {
       if (o.zone == 2) {
         ss.P = -(double)o.w / sh;
         ss.lg = -(double)o.w2 / sh;
@@ -188,7 +188,7 @@ __device__ __forceinline__ StageF make_stage_f(const DevScene& S, const StageOut
     if constexpr (KIND == KERR_FF) {
       ss.P = q * float(S.ff_C) * o.w * rcp_<true>(o.sh * o.sh * o.sh);
       ss.lg = log10_(o.sh);
-    } else if constexpr (KIND == POWEHI) {
+    } else if constexpr (plaw_v<KIND>) {
       if (o.zone == 2) {
         const float i2 = q * q;  // 1 / shift^2
         ss.P = -o.w * (i2 * i2);         // -J0 / shift^4
@@ -235,7 +235,7 @@ __device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, 
     T base = T(ss.P) * T(c.if3) * g;
     kT = (double)(base * (T(1) - em));
     kI = (double)(base * ES * exp_(-u - T(tau)));
-  } else if constexpr (KIND == POWEHI) {
+  } else if constexpr (plaw_v<KIND>) {
     if (ss.q == 0.0) {  // disk: a = const, j = a B_nu(T)
       kT = ss.P;
       kI = ss.P * EMISSION_SCALE * exp(-tau) / (exp(c.uf * ss.sh) - 1.0);
@@ -289,7 +289,7 @@ struct Integrator {
   // vector is gathered where a step needs it), plus r and phi for the stop test.
   double Ro[GM];      // owned components of [t r theta phi k^t k^r k^theta k^phi]
   double LS[GM];      // owned components of the last slope (k1)
-  double rr, ph;      // r and phi of the state
+  double rr, ph;      // r and phi of the state (pos_r, pos_phi)
   double sI[CMAX], sT[CMAX], lI[CMAX], lT[CMAX];
   double h;
   bool rejected;
@@ -312,8 +312,8 @@ struct Integrator {
   __device__ __forceinline__ void own(const double* X) {
 #pragma unroll
     for (int g = 0; g < GM; ++g) Ro[g] = sel8(X, lane + g * L);
-    rr = X[1];
-    ph = X[3];
+    rr = pos_r<KIND>(X);
+    ph = pos_phi<KIND>(X);
   }
 
   // Full derivative at state R: owned geodesic slope + owned colours' slopes
@@ -407,7 +407,7 @@ struct Integrator {
     bool done = false;
 #pragma unroll 1
     for (int it = 0; it <= 36; ++it) {
-      const bool term = stop_test(R[1], R[3], 2, 1);
+      const bool term = stop_test(pos_r<KIND>(R), pos_phi<KIND>(R), 2, 1);
       if (term) {
         if (it == 36) { done = true; break; }
         continue;
@@ -457,7 +457,7 @@ struct Integrator {
         const float base = ss.P * if3 * g;
         kTm = base * (1.0f - __expf(-u));
         kIm = base * ES * __expf(-u - (float)tau);
-      } else if constexpr (KIND == POWEHI) {
+      } else if constexpr (plaw_v<KIND>) {
         // disk: blackbody as the Kerr scene; jet: t = log2(nu / (nu_c sin)) = log2 f + q,
         // a / shift = A0 2^(e_a t) / (f shift^2), j / (shift nu^3) = J0 2^(e_j t) / (f^3 shift^4)
         const float t = lc + ss.q;
@@ -746,8 +746,8 @@ struct Integrator {
     }
     double R[8];
     gather(Ro, R);
-    rr = R[1];
-    ph = R[3];
+    rr = pos_r<KIND>(R);
+    ph = pos_phi<KIND>(R);
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) {
       lI[m] = 0;
@@ -769,7 +769,7 @@ struct Integrator {
     const long long pose = ray / A.npix;
     double b11[11];
     if (A.O.bundles) bundle_slots(A.cams[pose], ray - pose * A.npix, A.x_pix, b11);
-    if constexpr (KIND == WORMHOLE) {  // skybox_handling! (LttM.jl:54)
+    if constexpr (KIND == WORMHOLE || KIND == MINK_CART) {  // skybox_handling! (LttM.jl:54)
       double R[8], fs, w[3];
       gather(Ro, R);
       SkyBeam B;

@@ -160,8 +160,8 @@ vacuum_kernel(const KArgs* __restrict__ Ap, const PassArgs Q) {
     h = nh;
     rej = !accept;
     // calc_terminate without the optical-depth test (frozen tau was not thick)
-    if ((double)att >= max_steps || radial<KIND>(S, R[1]) > S.r_escape || (KIND != MINK && R[1] <= S.r_horizon) ||
-        R[3] > 8 * PI) {
+    if ((double)att >= max_steps || radial<KIND>(S, pos_r<KIND>(R)) > S.r_escape ||
+        (KIND != MINK && R[1] <= S.r_horizon) || pos_phi<KIND>(R) > 8 * PI) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) P.geo[ray * 8 + j] = R[j];
       P.h[ray] = h;
@@ -189,13 +189,14 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
   double X = 0, Y = 0, Z = 0;
   bool thick = A.O.ncol > 0;  // the class of the final state (bundles only change when rays stop)
   const bool wormhole = A.S.kind == WORMHOLE;
+  const bool cart = A.S.kind == MINK_CART || A.S.kind == SYNCH_TEST;
   double fs = 0, w[3];
   bool sky = false, beam = false;
   SkyBeam B;
   const long long pose = ray / A.npix;
   double b11[11];
   if (A.O.bundles) bundle_slots(A.cams[pose], ray - pose * A.npix, A.x_pix, b11);
-  if (wormhole) {  // skybox_handling! (LttM.jl:54)
+  if (wormhole || A.S.kind == MINK_CART) {  // skybox_handling! (LttM.jl:54)
     double R[8];
     for (int j = 0; j < 8; ++j) R[j] = P.geo[ray * 8 + j];
     if (A.O.bundles) beam = skybox_beam(A.S, A.sky, R, b11, B);
@@ -225,6 +226,7 @@ static __global__ void __launch_bounds__(256) finish_kernel(const KArgs* __restr
     if (!(v.y >= A.O.tau_thr)) thick = false;
   }
   double r = P.geo[ray * 8 + 1];
+  if (cart) r = sqrt(sumsq3(P.geo[ray * 8 + 1], P.geo[ray * 8 + 2], P.geo[ray * 8 + 3]));
   int cls;
   if (!isfinite(r)) cls = CLS_BOUNDARY;
   else if (A.S.kind != MINK && r <= A.S.r_horizon) cls = CLS_HORIZON;

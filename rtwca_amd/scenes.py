@@ -169,6 +169,31 @@ def wormhole():
     return s
 
 
+def minkowski_cart_sphere(temperature=5778.0, absorb=1.0, radius=1.0):
+    """main/Minkowski_Sphere_Cartesian.source: flat space in (t, x, y, z), the unit
+    sphere about the origin, a_nu = 1, T = 5778 K (source_model.jl:51-89); rays stop
+    100 from the origin (:158-172) and every ray adds skybox 1 in its direction
+    (:406-431; needs skybox 1 loaded)."""
+    s = _base(abi.SCENE_MINKOWSKI_CART_SPHERE)
+    s.temperature = temperature
+    s.absorb = absorb
+    s.r_out = radius
+    s.r_escape = 100.0
+    return s
+
+
+def synchro_test(bfield=10.0, radius=1.0):
+    """main/Synchro_Test.source: flat space in (t, x, y, z), a static cylinder
+    sqrt(x^2 + y^2) <= 1 (source_model.jl:216-223) in the uniform field B^z = 10 G
+    (:50-56) radiating power-law synchrotron (p = 2.5, gamma 30 .. 1e9, with the
+    anisotropy of the M87* jet; :152-213); rays stop 100 from the origin."""
+    s = _base(abi.SCENE_SYNCHRO_TEST)
+    s.bfield = bfield
+    s.r_out = radius
+    s.r_escape = 100.0
+    return s
+
+
 def kerr_synch(spin=0.94, r_in=6.0, r_max=12.0, density=1e45, jet_ratio=1e-4, jet_half_angle_deg=10.0,
                r_jet=10.0, bfield=10.0, theta_e=10.0):
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Fishbone-Moncrief
@@ -230,6 +255,10 @@ def initialize_world(name):
         return wormhole(), [0, 5, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
     if n == "minkowski_t,r,θ,ϕ_sphere":
         return minkowski_sphere(), [0, 4, PI / 2, 0], [PI / 2, 0], [0.0, 0.0, -PI / 2], 0.0
+    if n == "synch":  # initialization.jl:120-128
+        return synchro_test(), [0.0, 0.0, 0.0, 0.0], [0.0, 0.0], [PI / 8, 0.0, PI], 0.0
+    if n == "minkowski_t,x,y,z_sphere":  # initialization.jl:130-138
+        return minkowski_cart_sphere(), [0.0, 0.0, 0.0, -4.0], [0.0, 0.0], [0.0, 0.0, 0.0], 0.0
     raise ValueError("scene %r is not provided by rtwca_amd" % name)
 
 

@@ -1,6 +1,7 @@
 """Generate the committed fixtures under tests/golden/ from the oracle.
 
-    python tests/golden/make_golden.py [--flops-only | --synch-only | --m87-only | --wormhole-only]
+    python tests/golden/make_golden.py [--flops-only | --synch-only | --m87-only | --wormhole-only |
+                                        --cartesian-only]
 
 * oracle_<cfg>_<npix>px_<nlam>[_<n>poses].npz: small renders of the benchmark workloads
   (XYZ, class, attempted/accepted steps, final states) -- the golden vectors that the
@@ -137,6 +138,55 @@ def make_bundle_fixtures():
     print("wrote", name, "steps", int(r["steps"][:, 0].sum()), "classes", np.bincount(r["cls"], minlength=5))
 
 
+CART_FIXTURE = (24, 32)  # pixels across, sky face size
+
+
+def cart_workload(x_pix=24, sky_n=32, bundles=False):
+    """The reference's cart_sph test (initialization.jl:349-356: 350:30:750 nm, max_dt_scale
+    0.1, max_steps 1e4, gen_image's default tolerance 1e-4 and FOV) on a synthetic skybox 1
+    of face size sky_n (the reference's Cubemap_Sky_22 is not in its repository)."""
+    from rtwca_amd import abi, skybox
+    sc, pos, d, p, beta = scenes.initialize_world("minkowski_t,x,y,z_sphere")
+    cam = abi.make_camera(pos, d, beta, p, x_pix, 85, 60)
+    it = abi.make_integrator(1e-4, 1e-4, 0.1, 1e4, ray_bundles=bundles)
+    lam = [float(x) for x in range(350, 751, 30)]
+    return sc, cam, it, lam, skybox.default_skyboxes(sky_n)[0]
+
+
+def cart_fixture_name(bundles=False):
+    return os.path.join(HERE, "oracle_cart_sph_%dpx_sky%d%s.npz" % (CART_FIXTURE + ("_bundles" if bundles else "",)))
+
+
+def synchro_workload(x_pix=16):
+    """main/Synchro_Test.source from initialize_world("synch") (initialization.jl:120-128)
+    at gen_image's defaults (tolerance 1e-4, FOV 85 x 60) over 350:30:740 nm."""
+    from rtwca_amd import abi
+    sc, pos, d, p, beta = scenes.initialize_world("synch")
+    cam = abi.make_camera(pos, d, beta, p, x_pix, 85, 60)
+    it = abi.make_integrator(1e-4, 1e-4, 0.1, 1e4)
+    return sc, cam, it, [float(x) for x in range(350, 741, 30)]
+
+
+def synchro_fixture_name():
+    return os.path.join(HERE, "oracle_synchro_test_16px.npz")
+
+
+def make_cartesian_fixtures():
+    for bundles in (False, True):
+        sc, cam, it, lam, sky = cart_workload(*CART_FIXTURE, bundles=bundles)
+        O.load_skybox(1, sky)
+        r = O.render(sc, cam, it, lam, nthreads=os.cpu_count() or 1, want_rays=True)
+        np.savez_compressed(cart_fixture_name(bundles), xyz=r["xyz"], cls=r["cls"], steps=r["steps"],
+                            rays=r["rays"], lam=np.array(lam))
+        print("wrote", cart_fixture_name(bundles), "steps", int(r["steps"][:, 0].sum()),
+              "classes", np.bincount(r["cls"], minlength=5))
+    sc, cam, it, lam = synchro_workload()
+    r = O.render(sc, cam, it, lam, nthreads=os.cpu_count() or 1, want_rays=True)
+    np.savez_compressed(synchro_fixture_name(), xyz=r["xyz"], cls=r["cls"], steps=r["steps"], rays=r["rays"],
+                        lam=np.array(lam))
+    print("wrote", synchro_fixture_name(), "steps", int(r["steps"][:, 0].sum()))
+
+
 def make_flops():
     out = {"definition": "oracle counting scalar: add/sub/mul/div/sqrt = 1, transcendental call = 1, "
                          "no fused multiply-add; per attempted RK45 step, averaged over a pixel sample"}
@@ -175,10 +225,14 @@ if __name__ == "__main__":
         make_wormhole_fixture()
         make_bundle_fixtures()
         sys.exit(0)
+    elif "--cartesian-only" in sys.argv:
+        make_cartesian_fixtures()
+        sys.exit(0)
     elif "--flops-only" not in sys.argv:
         make_fixtures()
         make_synch_fixture()
         make_m87_fixture()
         make_wormhole_fixture()
         make_bundle_fixtures()
+        make_cartesian_fixtures()
     make_flops()
