@@ -47,7 +47,7 @@ TRAFFIC_SUMMARY = {3: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_thick_ff_
                    5: os.path.join(ROOT, "profiles", "rk45_kernel_kerr_synch_64x512_mixed_hbm.md")}
 WORKLOADS = {"kerr": 3, "synch": 5, "reference-suite": 0, "skybox": 7}
 
-# The reference's own timed tests (main/initialization.jl:352-512: 500 px wide, FOV 85 x
+# The reference's own timed tests (main/initialization.jl:349-512: 500 px wide, FOV 85 x
 # 60, 350:30:740 nm, max_dt_scale 0.1, max_steps 1e4) and the times it recorded
 # (main/Test Results/TestResults.csv; "5P" = CUDA.jl Float32 on an unstated NVIDIA GPU,
 # "LttM" = Float64 on the CPU with all threads).

@@ -281,7 +281,7 @@ struct Integrator {
   const float* sgc;      // LDS, weighted single Gaunt column (FP32 stages)
   StageSc* sst;          // LDS, 6 stage slots of this group
   StageF* sstf;          // LDS, the same for FP32 stages
-  double* cand;          // LDS candidate area (CMAX*4 per lane, stride BLOCK)
+  double* cand;          // LDS candidate next slopes (CMAX*2 per lane, stride BLOCK)
   int ncol;
   int gok;               // Gaunt column pair inside the table
 
@@ -302,7 +302,10 @@ struct Integrator {
         ncol(a.O.ncol), gok(a.S.g_ok_j) {}
 
   __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < ncol; }
-  __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 4 + q) * BLOCK + tid]; }
+  __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 2 + q) * BLOCK + tid]; }
+  // The candidate of a colour: the stage-7 point y = y0 + h next_slope (the same
+  // expression the error test used), so only next_slope is kept until the decision.
+  static __device__ __forceinline__ double cand_y(double y0, double hh, double n) { return y0 + hh * n; }
 
   __device__ __forceinline__ void gather(const double* xo, double* X) const {
 #pragma unroll
@@ -444,7 +447,7 @@ struct Integrator {
     double nT = 0, yT = 0;
 #pragma unroll
     for (int s = 2; s <= 7; ++s) {
-      const double tau = T0 + hh * aT[s];
+      const double tau = cand_y(T0, hh, aT[s]);
       if (s == 7) { nT = aT[7]; yT = tau; }
       const StageF ss = ssr[s - 2];
       // Branch-free: matter slopes are computed for every stage and a vacuum stage
@@ -500,14 +503,14 @@ struct Integrator {
       for (int u = s + 1; u <= 7; ++u)
         if (DPC[s][u - s] != 0) aT[u] += DPC[s][u - s] * kT;
     }
-    const double yI = I0 + hh * nI;
+    const double yI = cand_y(I0, hh, nI);
     const double dT = yT - T0 - hh * eT;
     const double dI = yI - I0 - hh * eI;
     // error terms: the reciprocal's last-bit error is immaterial to the controller
     const double qT = dT * __builtin_amdgcn_rcp(atol + fmax(fabs(T0), fabs(yT)) * rtol);
     const double qI = dI * __builtin_amdgcn_rcp(atol + fmax(fabs(I0), fabs(yI)) * rtol);
     e2 += qI * qI + qT * qT;
-    CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+    CAND(m, 0) = nI; CAND(m, 1) = nT;
   }
 
   // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
@@ -602,13 +605,13 @@ struct Integrator {
       for (int m = 0; m < CMAX; ++m) {
         if (!col_ok(m)) continue;
         double nT = lT[m] * B1, nI = lI[m] * B1;
-        double yT = sT[m] + hh * nT, yI = sI[m] + hh * nI;
+        double yT = cand_y(sT[m], hh, nT), yI = cand_y(sI[m], hh, nI);
         double dT = (yT - sT[m]) - hh * (lT[m] * DPE[1]);
         double dI = (yI - sI[m]) - hh * (lI[m] * DPE[1]);
         double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
         double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
         e2 += qI * qI + qT * qT;
-        CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+        CAND(m, 0) = nI; CAND(m, 1) = nT;
       }
     } else if (MIXED && !f64) {
       n_transfer++;
@@ -632,7 +635,7 @@ struct Integrator {
         double nT = 0, yT = 0;
 #pragma unroll 1
         for (int s = 2; s <= 7; ++s) {
-          double tau = sT[m] + hh * aT[0];
+          double tau = cand_y(sT[m], hh, aT[0]);
           if (s == 7) { nT = aT[0]; yT = tau; }
           const StageSc ss = sst[s - 2];
           double kT, kI;
@@ -646,13 +649,13 @@ struct Integrator {
           for (int t = 0; t < 5; ++t) aT[t] = aT[t + 1];
           aT[5] = 0;
         }
-        double yI = sI[m] + hh * nI;
+        double yI = cand_y(sI[m], hh, nI);
         double dT = yT - sT[m] - hh * eT;
         double dI = yI - sI[m] - hh * eI;
         double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
         double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
         e2 += qI * qI + qT * qT;
-        CAND(m, 0) = yI; CAND(m, 1) = yT; CAND(m, 2) = nI; CAND(m, 3) = nT;
+        CAND(m, 0) = nI; CAND(m, 1) = nT;
       }
     }
 
@@ -683,7 +686,9 @@ struct Integrator {
 #pragma unroll
       for (int m = 0; m < CMAX; ++m) {
         if (!col_ok(m)) continue;
-        sI[m] = CAND(m, 0); sT[m] = CAND(m, 1); lI[m] = CAND(m, 2); lT[m] = CAND(m, 3);
+        lI[m] = CAND(m, 0); lT[m] = CAND(m, 1);
+        sI[m] = cand_y(sI[m], hh, lI[m]);
+        sT[m] = cand_y(sT[m], hh, lT[m]);
       }
     }
     h = nh;
@@ -854,7 +859,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   float2* sg = reinterpret_cast<float2*>(scol + ncol_pad);
   StageSc* sstage = reinterpret_cast<StageSc*>(sg + 146);
   double* cand = reinterpret_cast<double*>(sstage + (BLOCK / L) * 6);
-  StageF* sstf = reinterpret_cast<StageF*>(cand + (size_t)CMAX * 4 * BLOCK);
+  StageF* sstf = reinterpret_cast<StageF*>(cand + (size_t)CMAX * 2 * BLOCK);
   float* sgc = reinterpret_cast<float*>(sstf + (BLOCK / L) * 6);
   for (int i = threadIdx.x; i < ncol; i += BLOCK) scol[i] = A.colors[i];
   if (KIND == KERR_FF) {
@@ -934,7 +939,10 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
 
 template <int L, int CMAX> constexpr size_t rk45_lds_bytes(int ncol) {
   return sizeof(DevColor) * (size_t)((ncol + 1) & ~1) + sizeof(float2) * 146 + sizeof(StageSc) * (BLOCK / L) * 6 +
-         sizeof(double) * (size_t)CMAX * 4 * BLOCK + sizeof(StageF) * (BLOCK / L) * 6 + sizeof(float) * 146;
+         sizeof(double) * (size_t)CMAX * 2 * BLOCK + sizeof(StageF) * (BLOCK / L) * 6 + sizeof(float) * 146;
 }
+// Two blocks (two waves per SIMD, the register budget of RTW_WAVES_PER_EU = 2) must fit
+// in a CU's 160 KB of LDS at 64 colours: one block per CU runs the kernel ~45 % slower.
+static_assert(2 * rk45_lds_bytes<8, 8>(64) <= 160 * 1024, "lane-group kernel LDS exceeds half a CU at 64 colours");
 
 }  // namespace rtw
