@@ -790,14 +790,17 @@ __device__ __forceinline__ void gaunt_slopes(float logu, const float2* col, floa
 // The same lookup for FP32 stages, on the single column gc[k] = w0 col[k].x +
 // w1 col[k].y (the column weights folded in once per render: equal to the two-
 // column form up to float rounding), with clamped indices instead of a branch.
-__device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, int ok_j) {
-  const float frac_i = gaunt_row_coord(logu);
+// frac_i is the row coordinate (log10 u + 16) / 0.2.
+__device__ __forceinline__ float gaunt_lookup_frac_f32(float frac_i, const float* gc, int ok_j) {
   const float ffl = floorf(frac_i), fce = ceilf(frac_i + 1e-3f);
   const int fi = (int)ffl, ci = (int)fce;
   const bool ok = ok_j && fi >= 1 && ci <= 146;
   const int li = min(max(fi, 1), 146) - 1, hi = min(max(ci, 1), 146) - 1;
   const float g = (fce - frac_i) * gc[li] + (frac_i - ffl) * gc[hi];
   return ok ? g : 1.2f;
+}
+__device__ __forceinline__ float gaunt_lookup_f32(float logu, const float* gc, int ok_j) {
+  return gaunt_lookup_frac_f32(gaunt_row_coord(logu), gc, ok_j);
 }
 
 // Skybox lookup (Wormhole.source/source_model.jl:150-276, 393-443; Minkowski_Sphere_

@@ -47,6 +47,9 @@ constexpr double DPC[8][6] = {
 constexpr double DPE[8] = {0, 5179.0 / 57600, 0, 7571.0 / 16695, 393.0 / 640, -92097.0 / 339200, 187.0 / 2100,
                               1.0 / 40};
 constexpr double B1 = 35.0 / 384;
+// b_s - b*_s, the weight of k_s in the embedded error estimate h sum (b - b*) k.
+constexpr double dp_b(int s) { return s == 1 ? B1 : (s >= 2 && s <= 6 ? DPC[s][7 - s] : 0.0); }
+constexpr double dp_err(int s) { return dp_b(s) - DPE[s]; }
 
 // k1 contributions to stages 2..7 (slot t <-> stage 2 + t; stage 7 is b1).
 __device__ __forceinline__ double k1c(int t) {
@@ -430,36 +433,43 @@ struct Integrator {
     return false;
   }
 
-  // Transfer stages 2..7 of colour slot m with FP32 slopes and FP64 stage sums,
-  // error terms and candidate (integrators_LttM.jl:109-185 restricted to the two
-  // components of the colour).  The stages are unrolled so the tableau entries
-  // are immediates, zero entries drop out and each stage sum is its own register.
+  // Transfer stages 2..7 of colour slot m in FP32 (integrators_LttM.jl:109-185
+  // restricted to the two components of the colour): slopes, stage sums and the
+  // error sum h sum (b - b*) k are FP32 (the stage optical depth only feeds FP32
+  // exponentials; the sums carry 2^-24 of the step's increment, far below rtol of
+  // the state), the candidate y = y0 + h next_slope and the error norm are FP64.
+  // The stages are unrolled so the tableau entries are immediates, zero entries drop
+  // out and each stage sum is its own register.
   __device__ __forceinline__ void colour_f32(int m, const StageF* ssr, double hh, double atol, double rtol,
                                              double& e2) {
     const DevColor& c = scol[lane + m * L];
     const float uf = (float)c.uf, lc = (float)c.lc, if3 = (float)c.if3;
     const float ES = float(EMISSION_SCALE);
-    const double T0 = sT[m], I0 = sI[m], kT1 = lT[m], kI1 = lI[m];
-    double aT[8];  // aT[u]: tau slope sum of stage u
+    const double T0 = sT[m], I0 = sI[m];
+    const float kT1 = (float)lT[m], kI1 = (float)lI[m], hf = (float)hh, T0f = (float)T0;
+    // Gaunt row coordinate (log10 u + 16) / 0.2 = 5 (log10 uf + 16) + 5 log10 shift
+    const float gA = (float)(5.0 * (c.lc + 16.0));
+    // exponents in base 2: u log2(e) = (uf log2(e)) shift, tau log2(e) from prescaled sums
+    constexpr double LOG2E = 1.4426950408889634;
+    const float uf2 = (float)(c.uf * LOG2E), hf2 = (float)(hh * LOG2E), T0f2 = (float)(T0 * LOG2E);
+    float aT[8];  // aT[u]: tau slope sum of stage u
 #pragma unroll
-    for (int u = 2; u <= 7; ++u) aT[u] = k1c(u - 2) * kT1;
-    double nI = kI1 * B1, eT = kT1 * DPE[1], eI = kI1 * DPE[1];
-    double nT = 0, yT = 0;
+    for (int u = 2; u <= 7; ++u) aT[u] = (float)k1c(u - 2) * kT1;
+    float nI = kI1 * (float)B1, eT = kT1 * (float)dp_err(1), eI = kI1 * (float)dp_err(1);
 #pragma unroll
     for (int s = 2; s <= 7; ++s) {
-      const double tau = cand_y(T0, hh, aT[s]);
-      if (s == 7) { nT = aT[7]; yT = tau; }
+      const float tau = fmaf(hf, aT[s], T0f);
       const StageF ss = ssr[s - 2];
       // Branch-free: matter slopes are computed for every stage and a vacuum stage
       // selects the zero slopes of the reference, so all stages of all colour
       // slots form one basic block the scheduler can interleave.
       float kTm, kIm;
       if constexpr (KIND == KERR_FF) {
-        const float u = uf * ss.sh;
-        const float g = gaunt_lookup_f32(lc + ss.lg, sgc, gok);
+        const float u2 = uf2 * ss.sh, tau2 = fmaf(hf2, aT[s], T0f2);
+        const float g = gaunt_lookup_frac_f32(fmaf(5.0f, ss.lg, gA), sgc, gok);
         const float base = ss.P * if3 * g;
-        kTm = base * (1.0f - __expf(-u));
-        kIm = base * ES * __expf(-u - (float)tau);
+        kTm = base * (1.0f - __builtin_amdgcn_exp2f(-u2));
+        kIm = base * ES * __builtin_amdgcn_exp2f(-u2 - tau2);
       } else if constexpr (plaw_v<KIND>) {
         // disk: blackbody as the Kerr scene; jet: t = log2(nu / (nu_c sin)) = log2 f + q,
         // a / shift = A0 2^(e_a t) / (f shift^2), j / (shift nu^3) = J0 2^(e_j t) / (f^3 shift^4)
@@ -467,9 +477,9 @@ struct Integrator {
         const float yj = __builtin_amdgcn_exp2f(float(S.pw_ej) * t);
         const float ya = __builtin_amdgcn_exp2f(float(S.pw_ea) * t);
         const float ilam = __builtin_amdgcn_rcpf((float)c.f);
-        const float kTj = ss.lg * ya * ilam, kIj = ss.P * yj * (ilam * ilam * ilam) * __expf(-(float)tau);
+        const float kTj = ss.lg * ya * ilam, kIj = ss.P * yj * (ilam * ilam * ilam) * __expf(-tau);
         const float kTd = ss.P;
-        const float kId = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);
+        const float kId = __fdividef(ss.P * ES * __expf(-tau), __expf(uf * ss.sh) - 1.0f);
         const bool diskz = ss.q == -__builtin_inff();
         kTm = diskz ? kTd : kTj;
         kIm = diskz ? kId : kIj;
@@ -484,33 +494,32 @@ struct Integrator {
         const float x3 = fmaf(x3c, ss.lg, x3c);
         const float E = lc + ss.P - (x3c * 1.4426950408889634f) * ss.lg +
                         2.0f * __builtin_amdgcn_logf(x3 + float(SYN_2_11_12));
-        const float EI = E - (float)tau * 1.4426950408889634f;
+        const float EI = E - tau * 1.4426950408889634f;
         kTm = -__builtin_amdgcn_exp2f(E > 120.0f ? 120.0f : E) * expm1_(uf * ss.sh);
         kIm = -ES * __builtin_amdgcn_exp2f(EI > 120.0f ? 120.0f : EI);
       } else {
         kTm = ss.P;
-        kIm = __fdividef(ss.P * ES * __expf(-(float)tau), __expf(uf * ss.sh) - 1.0f);
+        kIm = __fdividef(ss.P * ES * __expf(-tau), __expf(uf * ss.sh) - 1.0f);
       }
       const bool vac = ss.vac != 0.0f;
-      const float kTf = vac ? ss.P : kTm, kIf = vac ? ss.P : kIm;
-      const double kT = kTf, kI = kIf;
-      if (DPE[s] != 0) {
-        eT += DPE[s] * kT;
-        eI += DPE[s] * kI;
+      const float kT = vac ? ss.P : kTm, kI = vac ? ss.P : kIm;
+      if (dp_err(s) != 0) {
+        eT = fmaf((float)dp_err(s), kT, eT);
+        eI = fmaf((float)dp_err(s), kI, eI);
       }
-      if (s <= 6 && DPC[s][7 - s] != 0) nI += DPC[s][7 - s] * kI;
+      if (s <= 6 && DPC[s][7 - s] != 0) nI = fmaf((float)DPC[s][7 - s], kI, nI);
 #pragma unroll
       for (int u = s + 1; u <= 7; ++u)
-        if (DPC[s][u - s] != 0) aT[u] += DPC[s][u - s] * kT;
+        if (DPC[s][u - s] != 0) aT[u] = fmaf((float)DPC[s][u - s], kT, aT[u]);
     }
-    const double yI = cand_y(I0, hh, nI);
-    const double dT = yT - T0 - hh * eT;
-    const double dI = yI - I0 - hh * eI;
+    const double nT = aT[7], nId = nI;
+    const double yT = cand_y(T0, hh, nT), yI = cand_y(I0, hh, nId);
+    const double dT = hh * (double)eT, dI = hh * (double)eI;
     // error terms: the reciprocal's last-bit error is immaterial to the controller
     const double qT = dT * __builtin_amdgcn_rcp(atol + fmax(fabs(T0), fabs(yT)) * rtol);
     const double qI = dI * __builtin_amdgcn_rcp(atol + fmax(fabs(I0), fabs(yI)) * rtol);
     e2 += qI * qI + qT * qT;
-    CAND(m, 0) = nI; CAND(m, 1) = nT;
+    CAND(m, 0) = nId; CAND(m, 1) = nT;
   }
 
   // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
