@@ -46,7 +46,7 @@ template <int KIND, int L, int CMAX, bool MIXED>
 hipError_t launch_one(const KArgs& A, const KArgs* dA, const PassArgs& Q, int num_cu, hipStream_t st,
                       LaunchInfo& li) {
   auto kern = rk45_kernel<KIND, L, CMAX, MIXED>;
-  size_t lds = rk45_lds_bytes<L, CMAX>(A.O.ncol);
+  size_t lds = rk45_lds_bytes<L, CMAX, MIXED>(A.O.ncol);
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   const long long gpb = BLOCK / L;
@@ -95,6 +95,12 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
   return hipGetLastError();
 }
 
+// Mixed precision with more than 32 colours runs 4 lanes per ray (16 colour slots per
+// lane): the geodesic stages, which every lane of a ray repeats, then cost half as much
+// per ray; fewer colours and FP64 keep 8 lanes per ray.
+#ifndef RTW_MIXED_L4
+#define RTW_MIXED_L4 1
+#endif
 #define RTW_DEFINE_LAUNCH_KIND(KIND, NAME)                                                                 \
   hipError_t NAME(int which, const KArgs& A, const KArgs* dA, const PassArgs& Q, bool mixed, int num_cu,  \
                   hipStream_t st, LaunchInfo& li) {                                                        \
@@ -102,6 +108,8 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
       if (mixed) return launch_vac<KIND, true>(A, dA, Q, num_cu, st, li);                                  \
       return launch_vac<KIND, false>(A, dA, Q, num_cu, st, li);                                            \
     }                                                                                                      \
+    if (mixed && RTW_MIXED_L4 && A.O.ncol > 32 && A.O.ncol <= 64)                                          \
+      return launch_one<KIND, 4, 16, true>(A, dA, Q, num_cu, st, li);                                      \
     if (mixed) return launch_cmax<KIND, 8, true>(A, dA, Q, num_cu, st, li);                                \
     return launch_cmax<KIND, 8, false>(A, dA, Q, num_cu, st, li);                                          \
   }

@@ -271,9 +271,15 @@ __device__ __forceinline__ void spec_eval(const DevScene& S, const StageSc& ss, 
 #define RTW_WAVES_PER_EU 2
 #endif
 
+// The type of the colours' last slopes (k1) and of their candidates: FP32 for the
+// mixed-precision kernel with 16 colour slots per lane (4 lanes per ray), where FP64
+// slopes would not fit the registers and LDS of two blocks per CU; FP64 otherwise.
+template <bool MIXED, int CMAX> using slope_t = typename std::conditional<MIXED && (CMAX >= 16), float, double>::type;
+
 template <int KIND, int L, int CMAX, bool MIXED>
 struct Integrator {
   static constexpr int GM = 8 / L;  // geodesic components per lane
+  using SlopeT = slope_t<MIXED, CMAX>;
 
   const KArgs& A;   // in global memory (read through the scalar cache)
   const DevScene& S;
@@ -284,7 +290,7 @@ struct Integrator {
   const float* sgc;      // LDS, weighted single Gaunt column (FP32 stages)
   StageSc* sst;          // LDS, 6 stage slots of this group
   StageF* sstf;          // LDS, the same for FP32 stages
-  double* cand;          // LDS candidate next slopes (CMAX*2 per lane, stride BLOCK)
+  SlopeT* cand;          // LDS candidate next slopes (CMAX*2 per lane, stride BLOCK)
   int ncol;
   int gok;               // Gaunt column pair inside the table
 
@@ -293,19 +299,29 @@ struct Integrator {
   double Ro[GM];      // owned components of [t r theta phi k^t k^r k^theta k^phi]
   double LS[GM];      // owned components of the last slope (k1)
   double rr, ph;      // r and phi of the state (pos_r, pos_phi)
-  double sI[CMAX], sT[CMAX], lI[CMAX], lT[CMAX];
+  double sI[CMAX], sT[CMAX];
+  SlopeT lI[CMAX], lT[CMAX];
   double h;
   bool rejected;
   bool last_vac = false;              // last step accepted with every stage in vacuum
   unsigned long long n_transfer = 0;  // steps that ran the full transfer stages
 
   __device__ Integrator(const KArgs& a, int lane_, int tid_, const DevColor* c, const float2* g, const float* gc,
-                        StageSc* st, StageF* stf, double* cd)
+                        StageSc* st, StageF* stf, SlopeT* cd)
       : A(a), S(a.S), lane(lane_), tid(tid_), scol(c), sg(g), sgc(gc), sst(st), sstf(stf), cand(cd),
         ncol(a.O.ncol), gok(a.S.g_ok_j) {}
 
   __device__ __forceinline__ bool col_ok(int m) const { return lane + m * L < ncol; }
-  __device__ __forceinline__ double& CAND(int m, int q) { return cand[(m * 2 + q) * BLOCK + tid]; }
+  __device__ __forceinline__ SlopeT& CAND(int m, int q) { return cand[(m * 2 + q) * BLOCK + tid]; }
+  // A next slope as it is kept (SlopeT); FP32 keeps NaN and saturates at the FP32 range.
+  static __device__ __forceinline__ double keep(double x) {
+    if constexpr (sizeof(SlopeT) == 4) {
+      const double c = x > 3.0e38 ? 3.0e38 : (x < -3.0e38 ? -3.0e38 : x);
+      return (double)(float)c;
+    } else {
+      return x;
+    }
+  }
   // The candidate of a colour: the stage-7 point y = y0 + h next_slope (the same
   // expression the error test used), so only next_slope is kept until the decision.
   static __device__ __forceinline__ double cand_y(double y0, double hh, double n) { return y0 + hh * n; }
@@ -337,7 +353,12 @@ struct Integrator {
     for (int g = 0; g < GM; ++g) LS[g] = sel8(dfull, lane + g * L);
 #pragma unroll
     for (int m = 0; m < CMAX; ++m) {
-      if (col_ok(m)) spec_eval<KIND, T>(S, ss, scol[lane + m * L], sT[m], sg, lT[m], lI[m]);
+      if (col_ok(m)) {
+        double kT, kI;
+        spec_eval<KIND, T>(S, ss, scol[lane + m * L], sT[m], sg, kT, kI);
+        lT[m] = (SlopeT)keep(kT);
+        lI[m] = (SlopeT)keep(kI);
+      }
     }
   }
   __device__ __forceinline__ void deriv_any(bool f64, const double* R, double* d) {
@@ -512,14 +533,14 @@ struct Integrator {
       for (int u = s + 1; u <= 7; ++u)
         if (DPC[s][u - s] != 0) aT[u] = fmaf((float)DPC[s][u - s], kT, aT[u]);
     }
-    const double nT = aT[7], nId = nI;
+    const double nT = keep(aT[7]), nId = keep(nI);
     const double yT = cand_y(T0, hh, nT), yI = cand_y(I0, hh, nId);
     const double dT = hh * (double)eT, dI = hh * (double)eI;
     // error terms: the reciprocal's last-bit error is immaterial to the controller
     const double qT = dT * __builtin_amdgcn_rcp(atol + fmax(fabs(T0), fabs(yT)) * rtol);
     const double qI = dI * __builtin_amdgcn_rcp(atol + fmax(fabs(I0), fabs(yI)) * rtol);
     e2 += qI * qI + qT * qT;
-    CAND(m, 0) = nId; CAND(m, 1) = nT;
+    CAND(m, 0) = (SlopeT)nId; CAND(m, 1) = (SlopeT)nT;
   }
 
   // One RKDP_Step_w_buffer! call.  Returns true if the step was accepted.
@@ -613,14 +634,14 @@ struct Integrator {
 #pragma unroll
       for (int m = 0; m < CMAX; ++m) {
         if (!col_ok(m)) continue;
-        double nT = lT[m] * B1, nI = lI[m] * B1;
+        double nT = keep(lT[m] * B1), nI = keep(lI[m] * B1);
         double yT = cand_y(sT[m], hh, nT), yI = cand_y(sI[m], hh, nI);
         double dT = (yT - sT[m]) - hh * (lT[m] * DPE[1]);
         double dI = (yI - sI[m]) - hh * (lI[m] * DPE[1]);
         double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
         double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
         e2 += qI * qI + qT * qT;
-        CAND(m, 0) = nI; CAND(m, 1) = nT;
+        CAND(m, 0) = (SlopeT)nI; CAND(m, 1) = (SlopeT)nT;
       }
     } else if (MIXED && !f64) {
       n_transfer++;
@@ -658,13 +679,18 @@ struct Integrator {
           for (int t = 0; t < 5; ++t) aT[t] = aT[t + 1];
           aT[5] = 0;
         }
+        if constexpr (sizeof(SlopeT) == 4) {  // the kept next slopes define the candidate
+          nT = keep(nT);
+          nI = keep(nI);
+          yT = cand_y(sT[m], hh, nT);
+        }
         double yI = cand_y(sI[m], hh, nI);
         double dT = yT - sT[m] - hh * eT;
         double dI = yI - sI[m] - hh * eI;
         double qT = dT / (atol + fmax(fabs(sT[m]), fabs(yT)) * rtol);
         double qI = dI / (atol + fmax(fabs(sI[m]), fabs(yI)) * rtol);
         e2 += qI * qI + qT * qT;
-        CAND(m, 0) = nI; CAND(m, 1) = nT;
+        CAND(m, 0) = (SlopeT)nI; CAND(m, 1) = (SlopeT)nT;
       }
     }
 
@@ -867,7 +893,8 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   const int ncol_pad = (ncol + 1) & ~1;
   float2* sg = reinterpret_cast<float2*>(scol + ncol_pad);
   StageSc* sstage = reinterpret_cast<StageSc*>(sg + 146);
-  double* cand = reinterpret_cast<double*>(sstage + (BLOCK / L) * 6);
+  using SlopeT = slope_t<MIXED, CMAX>;
+  SlopeT* cand = reinterpret_cast<SlopeT*>(sstage + (BLOCK / L) * 6);
   StageF* sstf = reinterpret_cast<StageF*>(cand + (size_t)CMAX * 2 * BLOCK);
   float* sgc = reinterpret_cast<float*>(sstf + (BLOCK / L) * 6);
   for (int i = threadIdx.x; i < ncol; i += BLOCK) scol[i] = A.colors[i];
@@ -946,12 +973,14 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(RTW_
   }
 }
 
-template <int L, int CMAX> constexpr size_t rk45_lds_bytes(int ncol) {
+template <int L, int CMAX, bool MIXED> constexpr size_t rk45_lds_bytes(int ncol) {
   return sizeof(DevColor) * (size_t)((ncol + 1) & ~1) + sizeof(float2) * 146 + sizeof(StageSc) * (BLOCK / L) * 6 +
-         sizeof(double) * (size_t)CMAX * 2 * BLOCK + sizeof(StageF) * (BLOCK / L) * 6 + sizeof(float) * 146;
+         sizeof(slope_t<MIXED, CMAX>) * (size_t)CMAX * 2 * BLOCK + sizeof(StageF) * (BLOCK / L) * 6 +
+         sizeof(float) * 146;
 }
 // Two blocks (two waves per SIMD, the register budget of RTW_WAVES_PER_EU = 2) must fit
 // in a CU's 160 KB of LDS at 64 colours: one block per CU runs the kernel ~45 % slower.
-static_assert(2 * rk45_lds_bytes<8, 8>(64) <= 160 * 1024, "lane-group kernel LDS exceeds half a CU at 64 colours");
+static_assert(2 * rk45_lds_bytes<8, 8, false>(64) <= 160 * 1024, "lane-group kernel LDS exceeds half a CU at 64 colours");
+static_assert(2 * rk45_lds_bytes<4, 16, true>(64) <= 160 * 1024, "lane-group kernel LDS exceeds half a CU at 64 colours");
 
 }  // namespace rtw
