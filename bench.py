@@ -359,7 +359,8 @@ def main():
                           "max": int(stp[:, 0].max().item())},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VALU, "traffic": traffic_per_launch(idx),
-                     "kernel": "rk45_kernel<%s, 8 lanes/ray, mixed> (lane-group transfer kernel)" % kind,
+                     "kernel": "rk45_kernel<%s, %d lanes/ray, mixed> (lane-group transfer kernel)" % (
+                         kind, st["lanes_per_ray"]),
                      "launches": st["group_launches"], "avg_launch_ms": st["group_ms"] / nl,
                      "flops_per_launch": fg / nl, "steps_per_launch": st["steps_group"] / nl,
                      "flops_per_transfer_step": F["flops_per_transfer_step"],

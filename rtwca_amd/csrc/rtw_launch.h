@@ -95,9 +95,10 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
   return hipGetLastError();
 }
 
-// Mixed precision with more than 32 colours runs 4 lanes per ray (16 colour slots per
-// lane): the geodesic stages, which every lane of a ray repeats, then cost half as much
-// per ray; fewer colours and FP64 keep 8 lanes per ray.
+// The free-free scene in mixed precision with more than 32 colours runs 4 lanes per ray
+// (16 colour slots per lane): the geodesic stages, which every lane of a ray repeats,
+// then cost half as much per ray (3 % faster at 1024^2 despite more register spills).
+// The synchrotron scene measured no gain from it; other cases keep 8 lanes per ray.
 #ifndef RTW_MIXED_L4
 #define RTW_MIXED_L4 1
 #endif
@@ -108,7 +109,7 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
       if (mixed) return launch_vac<KIND, true>(A, dA, Q, num_cu, st, li);                                  \
       return launch_vac<KIND, false>(A, dA, Q, num_cu, st, li);                                            \
     }                                                                                                      \
-    if (mixed && RTW_MIXED_L4 && A.O.ncol > 32 && A.O.ncol <= 64)                                          \
+    if (KIND == KERR_FF && mixed && RTW_MIXED_L4 && A.O.ncol > 32 && A.O.ncol <= 64)                      \
       return launch_one<KIND, 4, 16, true>(A, dA, Q, num_cu, st, li);                                      \
     if (mixed) return launch_cmax<KIND, 8, true>(A, dA, Q, num_cu, st, li);                                \
     return launch_cmax<KIND, 8, false>(A, dA, Q, num_cu, st, li);                                          \
