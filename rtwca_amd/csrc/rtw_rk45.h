@@ -7,7 +7,7 @@
 // the controller branches.
 //
 // Layout: a ray is integrated by a group of L lanes of one wavefront
-// (L = 8: 8 rays per wave64).  Lane l owns
+// (L = 8: 8 rays per wave64; L = 4: 16).  Lane l owns
 //   * geodesic components j with j % L == l (one component when L = 8), and
 //   * colours c = l + m*L, m < CMAX, i.e. (I/nu^3, tau) and their last slopes.
 // The geodesic state is also kept replicated in every lane; each RK stage is a
@@ -17,8 +17,9 @@
 // lanes read from LDS.  The transfer equations of one colour depend only on
 // those scalars and on tau, so each lane integrates its colours stage by stage
 // with forward-accumulated stage sums (the same left-to-right summation order
-// as the reference).  Candidate spectral states are staged in LDS and committed
-// only when the step is accepted; the error norm is a group reduction.
+// as the reference).  Candidate next slopes are staged in LDS and committed
+// (with the candidate state y0 + h next_slope) only when the step is accepted;
+// the error norm is a group reduction.
 //
 // Scheduling: SCHED_GRID gives each group one ray; SCHED_PERSISTENT keeps
 // groups resident and lets a group that finishes its ray pull the next one from
