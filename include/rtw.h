@@ -182,7 +182,10 @@ typedef struct rtw_integrator {
   int32_t precision;      /* rtw_precision */
   int32_t schedule;       /* rtw_schedule */
   double fp64_radius;     /* MIXED: steps starting at r < fp64_radius run in FP64 */
-  int32_t lanes_per_ray;  /* 0 = library default */
+  int32_t lanes_per_ray;  /* lanes of a wavefront per ray in the lane-group kernel: 0 = the
+                             library's choice (4 for the free-free scene in MIXED with more
+                             than 32 colours, else 8), 8, or 4 (MIXED only; FP32 colour
+                             slopes, 16 colour slots per lane) */
   int32_t ray_bundles;    /* gen_image's ray_bundles (initialization.jl:210-211, 246-324): 11
                              constant bundle slots follow the colours (out_rays rows are
                              8 + 2N + 11), the wormhole's skybox is sampled over the beam

@@ -657,6 +657,10 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   if (opts->precision != RTW_PREC_FP64 && opts->precision != RTW_PREC_MIXED)
     return fail(c, RTW_E_ARG, "unknown precision");
   if (opts->ray_bundles && !(opts->atol < 1)) return fail(c, RTW_E_ARG, "ray bundles need atol < 1");
+  if (opts->lanes_per_ray != 0 && opts->lanes_per_ray != 4 && opts->lanes_per_ray != 8)
+    return fail(c, RTW_E_ARG, "lanes_per_ray must be 0 (library choice), 4 or 8");
+  if (opts->lanes_per_ray == 4 && opts->precision != RTW_PREC_MIXED)
+    return fail(c, RTW_E_UNSUPPORTED, "4 lanes per ray is a mixed-precision layout");
   for (int k = 0; k < n_lambda; ++k)
     if (!(lam[k] > 0)) return fail(c, RTW_E_ARG, "wavelengths must be positive");
   const rtw_camera& c0 = cams[0];
@@ -708,6 +712,7 @@ static int render_enqueue(rtw_ctx* c, const rtw_scene* scene, const rtw_camera* 
   A.O.ncol = n_lambda;
   A.O.n = 8 + 2 * n_lambda;
   A.O.bundles = opts->ray_bundles ? 1 : 0;
+  A.O.lanes = opts->lanes_per_ray;
   A.cams = c->cams.as<DevCam>();
   A.x_pix = c0.x_pix;
   A.y_pix = ny;

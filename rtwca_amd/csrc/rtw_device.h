@@ -74,6 +74,7 @@ struct DevOpts {
   double rtol, atol, max_steps, tau_thr, fp64_radius;
   int ncol, n;
   int bundles;  // ray bundles: no opacity stop, beam-sampled skybox, 11 more out_rays rows
+  int lanes;    // lanes per ray of the lane-group kernel: 0 (the library's choice), 4 or 8
 };
 
 // Per-colour constants (staged in LDS).

@@ -95,13 +95,16 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
   return hipGetLastError();
 }
 
-// The free-free scene in mixed precision with more than 32 colours runs 4 lanes per ray
-// (16 colour slots per lane): the geodesic stages, which every lane of a ray repeats,
-// then cost half as much per ray (3 % faster at 1024^2 despite more register spills).
-// The synchrotron scene measured no gain from it; other cases keep 8 lanes per ray.
-#ifndef RTW_MIXED_L4
-#define RTW_MIXED_L4 1
-#endif
+// Lanes per ray of the lane-group kernel (rtw_integrator.lanes_per_ray; 0 = this
+// choice): the free-free scene in mixed precision with more than 32 colours runs 4 lanes
+// per ray (16 colour slots per lane, FP32 colour slopes): the geodesic stages, which every
+// lane of a ray repeats, then cost half as much per ray (3 % faster at 1024^2 despite
+// more register spills).  The synchrotron scene measured no gain from it; other cases run
+// 8 lanes per ray.  4 lanes is a mixed-precision layout (checked by the caller).
+inline int lanes_for(int kind, const DevOpts& O, bool mixed) {
+  if (O.lanes == 4 || O.lanes == 8) return O.lanes;
+  return (kind == KERR_FF && mixed && O.ncol > 32) ? 4 : 8;
+}
 #define RTW_DEFINE_LAUNCH_KIND(KIND, NAME)                                                                 \
   hipError_t NAME(int which, const KArgs& A, const KArgs* dA, const PassArgs& Q, bool mixed, int num_cu,  \
                   hipStream_t st, LaunchInfo& li) {                                                        \
@@ -109,8 +112,13 @@ hipError_t launch_vac(const KArgs& A, const KArgs* dA, const PassArgs& Q, int nu
       if (mixed) return launch_vac<KIND, true>(A, dA, Q, num_cu, st, li);                                  \
       return launch_vac<KIND, false>(A, dA, Q, num_cu, st, li);                                            \
     }                                                                                                      \
-    if (KIND == KERR_FF && mixed && RTW_MIXED_L4 && A.O.ncol > 32 && A.O.ncol <= 64)                      \
+    if (mixed && lanes_for(KIND, A.O, mixed) == 4) {                                                       \
+      if (A.O.ncol > 64) {                                                                                 \
+        li.msg = "too many colours for the lane layout";                                                   \
+        return hipErrorInvalidValue;                                                                       \
+      }                                                                                                    \
       return launch_one<KIND, 4, 16, true>(A, dA, Q, num_cu, st, li);                                      \
+    }                                                                                                      \
     if (mixed) return launch_cmax<KIND, 8, true>(A, dA, Q, num_cu, st, li);                                \
     return launch_cmax<KIND, 8, false>(A, dA, Q, num_cu, st, li);                                          \
   }
