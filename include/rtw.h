@@ -205,7 +205,7 @@ enum rtw_class {
 
 typedef struct rtw_stats {
   double kernel_ms;         /* device time of the integration kernel(s) */
-  double total_ms;          /* wall time of the call */
+  double total_ms;          /* wall time of the rtw_render call (0 from rtw_last_stats) */
   int64_t rays;
   int64_t steps_attempted;  /* sum over rays of attempted RK45 steps (LttM.jl:43-46) */
   int64_t steps_accepted;
