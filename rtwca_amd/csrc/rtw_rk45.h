@@ -456,10 +456,10 @@ struct Integrator {
   }
 
   // Transfer stages 2..7 of colour slot m in FP32 (integrators_LttM.jl:109-185
-  // restricted to the two components of the colour): slopes, stage sums and the
-  // error sum h sum (b - b*) k are FP32 (the stage optical depth only feeds FP32
-  // exponentials; the sums carry 2^-24 of the step's increment, far below rtol of
-  // the state), the candidate y = y0 + h next_slope and the error norm are FP64.
+  // restricted to the two components of the colour): slopes and stage sums are FP32
+  // (the stage optical depth only feeds FP32 exponentials; the sums carry 2^-24 of
+  // the step's increment, far below rtol of the state); the embedded error estimate
+  // h sum (b - b*) k, the candidate y = y0 + h next_slope and the error norm are FP64.
   // The stages are unrolled so the tableau entries are immediates, zero entries drop
   // out and each stage sum is its own register.
   __device__ __forceinline__ void colour_f32(int m, const StageF* ssr, double hh, double atol, double rtol,
@@ -477,7 +477,8 @@ struct Integrator {
     float aT[8];  // aT[u]: tau slope sum of stage u
 #pragma unroll
     for (int u = 2; u <= 7; ++u) aT[u] = (float)k1c(u - 2) * kT1;
-    float nI = kI1 * (float)B1, eT = kT1 * (float)dp_err(1), eI = kI1 * (float)dp_err(1);
+    float nI = kI1 * (float)B1;
+    double eT = (double)kT1 * dp_err(1), eI = (double)kI1 * dp_err(1);  // the error estimate in FP64
 #pragma unroll
     for (int s = 2; s <= 7; ++s) {
       const float tau = fmaf(hf, aT[s], T0f);
@@ -526,8 +527,8 @@ struct Integrator {
       const bool vac = ss.vac != 0.0f;
       const float kT = vac ? ss.P : kTm, kI = vac ? ss.P : kIm;
       if (dp_err(s) != 0) {
-        eT = fmaf((float)dp_err(s), kT, eT);
-        eI = fmaf((float)dp_err(s), kI, eI);
+        eT += dp_err(s) * (double)kT;
+        eI += dp_err(s) * (double)kI;
       }
       if (s <= 6 && DPC[s][7 - s] != 0) nI = fmaf((float)DPC[s][7 - s], kI, nI);
 #pragma unroll
@@ -536,7 +537,7 @@ struct Integrator {
     }
     const double nT = keep(aT[7]), nId = keep(nI);
     const double yT = cand_y(T0, hh, nT), yI = cand_y(I0, hh, nId);
-    const double dT = hh * (double)eT, dI = hh * (double)eI;
+    const double dT = hh * eT, dI = hh * eI;
     // error terms: the reciprocal's last-bit error is immaterial to the controller
     const double qT = dT * __builtin_amdgcn_rcp(atol + fmax(fabs(T0), fabs(yT)) * rtol);
     const double qI = dI * __builtin_amdgcn_rcp(atol + fmax(fabs(I0), fabs(yI)) * rtol);
