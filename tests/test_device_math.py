@@ -77,3 +77,29 @@ def test_controller_factor_matches_pow():
     with np.errstate(all="ignore"):
         naive = np.exp2(-0.2 * np.log2(err.astype(np.float32)).astype(np.float64))
     assert not np.all(np.abs(naive[ok] / ref[ok] - 1) < 3e-7)
+
+
+def _log_torus(x):
+    """The FP64 log of the torus potential in FP32 stages (rtw_device.h, log_torus): frexp
+    reduction to [sqrt(1/2), sqrt(2)) and the atanh series to s^15."""
+    m, e = np.frexp(x)
+    low = m < 0.70710678118654752
+    m = np.where(low, m + m, m)
+    e = np.where(low, e - 1, e)
+    s = (m - 1.0) / (m + 1.0)
+    z = s * s
+    p = np.full_like(x, 1.0 / 15)
+    for c in (1.0 / 13, 1.0 / 11, 1.0 / 9, 1.0 / 7, 1.0 / 5, 1.0 / 3, 1.0):
+        p = p * z + c
+    return e * 0.69314718055994531 + 2.0 * s * p
+
+
+def test_log_torus_accuracy():
+    # the argument (1 + q) A / (Sigma Delta) of the Fishbone-Moncrief potential spans
+    # ~1 .. 1e6 in the scenes; check 1e-300 .. 1e300 and dense sampling of [0.5, 4]
+    rng = np.random.default_rng(11)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 200000), rng.uniform(0.5, 4.0, 200000),
+                        np.array([1.0, 0.70710678118654752, 0.7071067811865476, 2.0, np.e])])
+    err = np.abs(_log_torus(x) - np.log(x))
+    assert err.max() < 3e-13, err.max()
+    assert _log_torus(np.array([1.0]))[0] == 0.0
