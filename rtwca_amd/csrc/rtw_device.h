@@ -177,6 +177,44 @@ __device__ __forceinline__ void bundle_slots(const DevCam& C, long long pix, int
 
 __device__ __forceinline__ void sc_(double x, double& s, double& c) { sincos(x, &s, &c); }
 __device__ __forceinline__ void sc_(float x, float& s, float& c) { sincosf(x, &s, &c); }
+// sin and cos of an FP32-stage angle: Cody-Waite reduction by pi/2 (pi/2 split into
+// three floats), Taylor polynomials of sin to r^9 and cos to r^10 on |r| <= pi/4
+// (truncation below 3e-9, under an FP32 ulp), quadrant by k mod 4.  Accurate to
+// about an ulp for |x| < 2^12 (the stage angles are theta in [0, pi] plus a step);
+// libm's sincosf also carries its large-argument reduction, which the compiler
+// evaluates for every call (~4 times the instructions).  NaN in, NaN out.
+__device__ __forceinline__ void sincos_stage(float x, float& s, float& c) {
+  constexpr double PIO2 = 1.5707963267948966;
+  constexpr float P1 = (float)PIO2;
+  constexpr float P2 = (float)(PIO2 - (double)P1);
+  constexpr float P3 = (float)(PIO2 - (double)P1 - (double)P2);
+  const float k = rintf(x * (float)(2.0 / 3.141592653589793));
+  float r = fmaf(-k, P1, x);
+  r = fmaf(-k, P2, r);
+  r = fmaf(-k, P3, r);
+  const float z = r * r;
+  float ps = 1.0f / 362880.0f;
+  ps = fmaf(ps, z, -1.0f / 5040.0f);
+  ps = fmaf(ps, z, 1.0f / 120.0f);
+  ps = fmaf(ps, z, -1.0f / 6.0f);
+  const float sr = fmaf(ps * z, r, r);
+  float pc = -1.0f / 3628800.0f;
+  pc = fmaf(pc, z, 1.0f / 40320.0f);
+  pc = fmaf(pc, z, -1.0f / 720.0f);
+  pc = fmaf(pc, z, 1.0f / 24.0f);
+  pc = fmaf(pc, z, -0.5f);
+  const float cr = fmaf(pc, z, 1.0f);
+  const int q = (int)k & 3;
+  const float s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
+  s = (q == 2 || q == 3) ? -s0 : s0;
+  c = (q == 1 || q == 2) ? -c0 : c0;
+}
+// The FP32 stages (F) use sincos_stage; FP64, the reference's Float32 Christoffel mode
+// and the adjoint's dual numbers keep sincos.
+template <bool F, class T> __device__ __forceinline__ void scf_(T x, T& s, T& c) {
+  if constexpr (F && sizeof(T) == 4) sincos_stage(x, s, c);
+  else sc_(x, s, c);
+}
 __device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float rsq_(float x) { return 1.0f / sqrtf(x); }
 __device__ __forceinline__ double exp_(double x) { return exp(x); }
@@ -270,7 +308,7 @@ template <class T> struct StageOut {
 template <class Gt, class T, bool FAST = false>
 __device__ __forceinline__ void kerr_accel(Gt r, Gt th, Gt a, Gt rs, const T k[4], T acc[4]) {
   Gt s, c;
-  sc_(th, s, c);
+  scf_<FAST>(th, s, c);
   Gt r2 = r * r, a2 = a * a, s2 = s * s, c2 = c * c;
   Gt ac2 = a2 * c2, as2 = a2 * s2;
   Gt S = r2 + ac2;
@@ -402,7 +440,7 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
   } else if constexpr (KIND == MINK) {
     // Minkowski_Sphere_Spherical.source/source_geometry.jl:93-108
     T s, c;
-    sc_(th, s, c);
+    scf_<F>(th, s, c);
     T ir = rcp_<F>(r + T(NO_DIV_ZERO));
     T cot = c * rcp_<F>(s);
     o.d[4] = T(0);
@@ -416,7 +454,7 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
     // Wormhole.source/source_geometry.jl:129-146; static observer (source_model.jl:111-118),
     // no matter anywhere
     T s, c;
-    sc_(th, s, c);
+    scf_<F>(th, s, c);
     const T rl = wh_r<T>(S, r), drdl = wh_drdl<T>(S, r);
     const T rr = rl * drdl, ird = drdl * rcp_<F>(rl);
     const T cot = c * rcp_<F>(s);
@@ -429,7 +467,7 @@ __device__ __forceinline__ void geo_rhs(const DevScene& S, const T x[8], StageOu
   } else if constexpr (KIND == SCHW) {
     // Schwarzschild_Simple_Disk_Spherical.source/source_geometry.jl:109-128
     T s, c;
-    sc_(th, s, c);
+    scf_<F>(th, s, c);
     const T rs = T(S.rs);
     T ir = rcp_<F>(r);
     T rmrs = r - rs;
@@ -483,7 +521,7 @@ __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T 
     }
     o.d[4] = acc[0]; o.d[5] = acc[1]; o.d[6] = acc[2]; o.d[7] = acc[3];
     T s, c;
-    sc_(th, s, c);
+    scf_<F>(th, s, c);
     const T rs = T(S.rs), M = T(S.M);
     T r2 = r * r, a2 = a * a, s2 = s * s;
     T Sg = r2 + a2 * c * c;

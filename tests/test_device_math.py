@@ -103,3 +103,47 @@ def test_log_torus_accuracy():
     err = np.abs(_log_torus(x) - np.log(x))
     assert err.max() < 3e-13, err.max()
     assert _log_torus(np.array([1.0]))[0] == 0.0
+
+
+def _fmaf(a, b, c):
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(np.float32)
+
+
+def _sincos_stage(x):
+    """The FP32-stage sin/cos (rtw_device.h, sincos_stage), FMAs emulated in float64."""
+    f = np.float32
+    pio2 = 1.5707963267948966
+    p1 = f(pio2)
+    p2 = f(pio2 - float(p1))
+    p3 = f(pio2 - float(p1) - float(p2))
+    x = x.astype(np.float32)
+    k = np.rint((x * f(2.0 / 3.141592653589793)).astype(np.float32)).astype(np.float32)
+    r = _fmaf(-k, np.full_like(x, p1), x)
+    r = _fmaf(-k, np.full_like(x, p2), r)
+    r = _fmaf(-k, np.full_like(x, p3), r)
+    z = (r * r).astype(np.float32)
+    ps = np.full_like(x, f(1.0 / 362880.0))
+    for cf in (-1.0 / 5040.0, 1.0 / 120.0, -1.0 / 6.0):
+        ps = _fmaf(ps, z, np.full_like(x, f(cf)))
+    sr = _fmaf((ps * z).astype(np.float32), r, r)
+    pc = np.full_like(x, f(-1.0 / 3628800.0))
+    for cf in (1.0 / 40320.0, -1.0 / 720.0, 1.0 / 24.0, -0.5):
+        pc = _fmaf(pc, z, np.full_like(x, f(cf)))
+    cr = _fmaf(pc, z, np.ones_like(x))
+    q = k.astype(np.int64) & 3
+    s0 = np.where(q & 1, cr, sr)
+    c0 = np.where(q & 1, sr, cr)
+    return np.where((q == 2) | (q == 3), -s0, s0), np.where((q == 1) | (q == 2), -c0, c0)
+
+
+def test_stage_sincos_accuracy():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-1.0, 4.2, 400000), rng.uniform(-4096, 4096, 100000),
+                        np.array([0.0, 1.5707964, 3.1415927, -3.1415927, 4.712389])]).astype(np.float32)
+    s, c = _sincos_stage(x)
+    xs = x.astype(np.float64)
+    for got, ref in ((s, np.sin(xs)), (c, np.cos(xs))):
+        err = np.abs(got.astype(np.float64) - ref)
+        assert err.max() < 1.2e-7, err.max()  # an FP32 ulp at 1
+        big = np.abs(ref) > 1e-3
+        assert (err[big] / np.abs(ref[big])).max() < 4e-7
