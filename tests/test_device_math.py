@@ -86,7 +86,7 @@ def _log_torus(x):
     low = m < 0.70710678118654752
     m = np.where(low, m + m, m)
     e = np.where(low, e - 1, e)
-    s = (m - 1.0) / (m + 1.0)
+    s = (m - 1.0) / (m + 1.0)  # the device forms 1/(m + 1) from v_rcp_f64 and a Newton step
     z = s * s
     p = np.full_like(x, 1.0 / 15)
     for c in (1.0 / 13, 1.0 / 11, 1.0 / 9, 1.0 / 7, 1.0 / 5, 1.0 / 3, 1.0):
