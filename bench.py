@@ -86,6 +86,8 @@ def parse():
                          "gradients vs central finite differences, and exit")
     ap.add_argument("--adjoint-x-pix", type=int, default=256)
     ap.add_argument("--adjoint-first", type=int, default=0, help="index of the first draw of --adjoint-check")
+    ap.add_argument("--adjoint-edge-width", type=float, default=None,
+                    help="disk edge width of --adjoint-check (default: the workload's softened edges; 0: hard)")
     return ap.parse_args()
 
 
@@ -126,26 +128,27 @@ def copy_probe(mib, torch):
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
 
 
-def adjoint_check(ndraw, x_pix, first=0):
+def adjoint_check(ndraw, x_pix, first=0, edge_width=None):
     """Differentiable Kerr workload (x_pix^2, 64 wavelengths, rtol = atol = 1e-8): for each
     draw p (uniform in the parameter boxes), L2 loss against an image at p perturbed by 5 %
     (normal, per parameter); adjoint gradient vs central differences with h = 1e-4 in each
     parameter's natural scale, of the recorded accepted-step map (frozen) and of fresh
     adaptive renders (adaptive).  Prints one JSON line."""
     from rtwca_amd import adjoint, host
+    ew = adjoint.EDGE_WIDTH if edge_width is None else edge_width
     ctx = host.Context(0)
     rows = []
     for i in range(first, first + ndraw):
         rng = np.random.default_rng([20240601, i])  # draw i does not depend on how the draws are split
         p = adjoint.draw(rng)
         ptgt = p * (1 + 0.05 * rng.standard_normal(4))
-        tgt = adjoint.render(ctx, ptgt, x_pix, 64)["xyz"]
-        r = adjoint.gradcheck(ctx, p, tgt, x_pix, 64)
+        tgt = adjoint.render(ctx, ptgt, x_pix, 64, edge_width=ew)["xyz"]
+        r = adjoint.gradcheck(ctx, p, tgt, x_pix, 64, edge_width=ew)
         rows.append(r)
-        print("  draw %d p=%s frozen rel %s adaptive rel %s kept %.2f adjoint %.1fs" %
-              (i, np.array2string(p, precision=4), np.array2string(r["rel_frozen"], precision=2),
-               np.array2string(r["rel_adaptive"], precision=2), r["kept_frozen"], r["adjoint_seconds"]),
-              file=sys.stderr, flush=True)
+        print("  draw %d p=%s adjoint %.1fs" % (i, np.array2string(p, precision=4), r["adjoint_seconds"]) +
+              "".join("\n    %-17s rel %s kept %.3f" % (t, " ".join("%.1e" % v for v in r["rel_" + t]), r["kept_" + t])
+                      for t in ("frozen", "resolved", "adaptive", "adaptive_resolved")
+                      if "rel_" + t in r), file=sys.stderr, flush=True)
 
     def summary(key):
         a = np.array([r[key] for r in rows])
@@ -153,15 +156,20 @@ def adjoint_check(ndraw, x_pix, first=0):
                        "max": float(a[:, k].max()), "within_1e-3": float((a[:, k] <= 1e-3).mean())}
                 for k, name in enumerate(adjoint.PARAMS)}
     print(json.dumps({"adjoint_check": {
-        "workload": "kerr_thick_ff_adjoint_%d" % x_pix, "draws": ndraw, "fd_step": "1e-4 (spin, inclination "
-        "absolute; temperature, density relative)", "frozen_rel_err": summary("rel_frozen"),
-        "adaptive_rel_err": summary("rel_adaptive"),
+        "workload": "kerr_thick_ff_adjoint_%d" % x_pix, "draws": ndraw, "edge_width": ew,
+        "fd_step": "1e-4 (spin, inclination absolute; temperature, density relative)",
+        "frozen_rel_err": summary("rel_frozen"), "adaptive_rel_err": summary("rel_adaptive"),
+        "resolved_rel_err": summary("rel_resolved"), "adaptive_resolved_rel_err": summary("rel_adaptive_resolved"),
+        "kept_pixels_resolved_median": float(np.median([r["kept_resolved"] for r in rows])),
         "kept_pixels_median": float(np.median([r["kept_frozen"] for r in rows])),
+        "kept_pixels_adaptive_median": float(np.median([r["kept_adaptive"] for r in rows])),
         "adjoint_seconds_median": float(np.median([r["adjoint_seconds"] for r in rows])),
         "steps_per_render_median": float(np.median([r["steps"] for r in rows])),
         "per_draw": [{"draw": first + i, "adjoint": r["adjoint_frozen"].tolist(), "fd_frozen": r["fd_frozen"].tolist(),
                       "adjoint_adaptive_set": r["adjoint_adaptive"].tolist(), "fd_adaptive": r["fd_adaptive"].tolist(),
-                      "kept": r["kept_frozen"]} for i, r in enumerate(rows)]}}))
+                      "fd_resolved": r["fd_resolved"].tolist(), "adjoint_resolved_set": r["adjoint_resolved"].tolist(),
+                      "kept": r["kept_frozen"], "kept_adaptive": r["kept_adaptive"], "kept_resolved": r["kept_resolved"]}
+                     for i, r in enumerate(rows)]}}))
 
 
 def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
@@ -314,7 +322,7 @@ def main():
         copy_probe(a.copy_probe_mib, torch)
         return
     if a.adjoint_check:
-        adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first)
+        adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first, a.adjoint_edge_width)
         return
     if a.workload == "reference-suite":
         reference_suite(a)

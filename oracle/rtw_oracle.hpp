@@ -168,6 +168,8 @@ struct Scene {
   double jet_half_angle = 0.0;       // synchrotron: jet opening half-angle (rad)
   double r_jet = 0.0;                // synchrotron: jet base radius
   double torus_r_max = 12.0;         // synchrotron: torus pressure maximum
+  double edge_width = 0.0;           // thick disk: radial edges softened over this width (0: hard)
+  double r_in_spin_slope = 0.0;      // thick disk: d r_in / d spin (adjoint only; unused here)
   // derived by scene_init
   double r_s = 2.0, r_ev = 0.0;
   // synchrotron torus + jet (scene_init): torus angular momentum u_phi u^t, its
@@ -777,6 +779,15 @@ template <class R> R freq_shift(const R* ray, const R u[4], const R g[4][4]) {
   return fs;
 }
 
+// Softened edges of the thick disk (no reference counterpart): the Gaussian less its
+// value at the vertical cut, times the radial window W(r) = s((r - r_in) / w)
+// s((r_out - r) / w) with the smoothstep s(t) = 3t^2 - 2t^3, s = 1 for t >= 1; rho
+// then vanishes at every edge and rho^2 has a continuous slope across it.
+template <class R> R edge_window(const Scene& s, R r) {
+  auto st = [](R t) { return t < R(1) ? t * t * (R(3) - R(2) * t) : R(1); };
+  return st((r - R(s.r_in)) / R(s.edge_width)) * st((R(s.r_out) - r) / R(s.edge_width));
+}
+
 // Per-point fluid parameters (calc_fluid_params!, Minkowski source_model.jl:222-226:
 // T only).  For the thick disk fluid[1] = rho^2 at the point; for synchrotron the
 // scene's own set.
@@ -786,6 +797,8 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
     if (is_fire(s, x)) {
       R c = cos(x[2]);
       R rho = R(s.density) * exp(-(c * c) / R(2 * s.h_over_r * s.h_over_r));
+      if (s.edge_width > 0)
+        rho = (rho - R(s.density * std::exp(-0.5 * s.z_cut * s.z_cut))) * edge_window(s, x[1]);
       fp[1] = rho * rho;
     } else {
       fp[1] = R(0);

@@ -24,6 +24,7 @@ static Scene to_scene(const rtw_scene* s) {
   o.h_over_r = s->h_over_r; o.z_cut = s->z_cut; o.r_escape = s->r_escape;
   o.bfield = s->bfield; o.theta_e = s->theta_e; o.jet_half_angle = s->jet_half_angle;
   o.r_jet = s->r_jet; o.torus_r_max = s->torus_r_max;
+  o.edge_width = s->edge_width; o.r_in_spin_slope = s->r_in_spin_slope;
   scene_init(o);
   return o;
 }

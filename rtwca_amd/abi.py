@@ -50,6 +50,7 @@ class Scene(C.Structure):
         ("h_over_r", C.c_double), ("z_cut", C.c_double), ("r_escape", C.c_double),
         ("bfield", C.c_double), ("theta_e", C.c_double), ("jet_half_angle", C.c_double),
         ("r_jet", C.c_double), ("torus_r_max", C.c_double),
+        ("edge_width", C.c_double), ("r_in_spin_slope", C.c_double),
     ]
 
 

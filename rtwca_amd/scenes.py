@@ -12,11 +12,13 @@ decided here, once:
   * cfg2 disk: the reference's theta wedge [15pi/32, 17pi/32] with a_nu = 1;
   * cfg3/4 thick disk: r from the reference-formula ISCO to 12, H = 0.1 r,
     truncated at |z| <= 4H, kappa = 0.2, tolerance rtol = atol = 1e-6 (cfg3),
-    1e-8 (cfg4);
+    1e-8 (cfg4); cfg4, the differentiable workload, softens the disk edges over
+    DIFF_EDGE_WIDTH so that its image is C^1 in the parameters;
   * cfg5 torus + jet: see `kerr_synch`; 64 static cameras at r = 50 on a circle
     tilted 60 degrees from the equator (`inclined_orbit_cameras`), FOV 60/60,
     rtol = atol = 1e-6, max_steps = 2e4.
 """
+import cmath
 import math
 from fractions import Fraction
 
@@ -54,6 +56,20 @@ def kerr_isco_ref(a, M=1.0, f32_mode=False):
     Z1 = 1 + (1 - chi * chi) ** (1 / 3) * ((1 + chi) ** 1.5 + (1 - chi) ** 1.5)
     Z2 = math.sqrt(3 * (chi * chi) + Z1 * Z1)
     return M * (3 + Z2 - math.sqrt((3 - Z1) * (3 + Z1 + 2 * Z2)))
+
+
+# radial edge width (M) of the softened thick disk of the differentiable workload
+DIFF_EDGE_WIDTH = 1.0
+
+
+def kerr_isco_ref_spin_slope(a, M=1.0):
+    """d/da of kerr_isco_ref(a, M) (FP64 form), by complex step (the formula is
+    analytic for 0 < a < M)."""
+    h = 1e-20
+    chi = 2 * complex(a, h) / (2 * M)
+    Z1 = 1 + (1 - chi * chi) ** (1 / 3) * ((1 + chi) ** 1.5 + (1 - chi) ** 1.5)
+    Z2 = cmath.sqrt(3 * (chi * chi) + Z1 * Z1)
+    return (M * (3 + Z2 - cmath.sqrt((3 - Z1) * (3 + Z1 + 2 * Z2)))).imag / h
 
 
 def kerr_circ_beta(r, a, M=1.0):
@@ -123,10 +139,13 @@ def kerr_disk(spin=0.998, ref_f32=True):
 
 
 def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, r_out=12.0,
-                  h_over_r=0.1, z_cut=4.0, gaunt_fp64=False):
+                  h_over_r=0.1, z_cut=4.0, gaunt_fp64=False, edge_width=0.0):
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Gaussian thick
     disk rho = rho_s exp(-z^2 / 2H^2), H = 0.1 r, |z| <= 4H, radiating free-free with the
-    van Hoof Gaunt factor (main/emission_functions.jl:54-78) and Kirchhoff's law."""
+    van Hoof Gaunt factor (main/emission_functions.jl:54-78) and Kirchhoff's law.
+    edge_width > 0 softens the edges (the Gaussian less its value at the cut, radial
+    smoothstep windows of that width); with the default inner edge (the
+    reference-formula ISCO of the spin) the scene also carries d r_in / d spin."""
     s = _base(abi.SCENE_KERR_THICK_FF)
     s.spin = spin
     s.temperature = temperature
@@ -136,6 +155,9 @@ def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, 
     s.r_out = r_out
     s.h_over_r = h_over_r
     s.z_cut = z_cut
+    s.edge_width = edge_width
+    if edge_width > 0 and r_in is None:
+        s.r_in_spin_slope = kerr_isco_ref_spin_slope(spin)
     if gaunt_fp64:
         s.flags |= abi.SCENE_GAUNT_FP64
     return s
@@ -298,7 +320,7 @@ def config(idx, x_pix=None, precision=abi.PREC_FP64, schedule=abi.SCHED_PERSISTE
         it = abi.make_integrator(1e-6, 1e-6, 0.1, 1e4, precision, schedule, 3.0)
         return Workload("kerr_a0.9_thick_ff_1024", sc, [cam], it, lam)
     if idx == 4:
-        sc = kerr_thick_ff(spin=0.9, r_in=kerr_isco_ref(0.9), gaunt_fp64=True)
+        sc = kerr_thick_ff(spin=0.9, gaunt_fp64=True, edge_width=DIFF_EDGE_WIDTH)
         cam = abi.make_camera([0, 10, 60 * PI / 180, 0], [0, 0], 0.0, [-PI / 2, 0, 0],
                               x_pix or 256, 60, 60)
         it = abi.make_integrator(1e-8, 1e-8, 0.1, 1e4, precision, schedule)

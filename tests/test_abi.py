@@ -45,7 +45,7 @@ def test_struct_layouts_match_c(tmp_path):
 #include "rtw.h"
 int main(void) {
   printf("%zu %zu %zu %zu\\n", sizeof(rtw_scene), sizeof(rtw_camera), sizeof(rtw_integrator), sizeof(rtw_stats));
-  printf("%zu %zu %zu %zu\\n", offsetof(rtw_scene, torus_r_max), offsetof(rtw_camera, fov_vert),
+  printf("%zu %zu %zu %zu\\n", offsetof(rtw_scene, r_in_spin_slope), offsetof(rtw_camera, fov_vert),
          offsetof(rtw_integrator, lanes_per_ray), offsetof(rtw_stats, lanes_per_ray));
   return 0;
 }
@@ -54,7 +54,7 @@ int main(void) {
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)])
     sizes, offs = [list(map(int, l.split())) for l in subprocess.check_output([str(exe)]).decode().split("\n")[:2]]
     assert sizes == [C.sizeof(abi.Scene), C.sizeof(abi.Camera), C.sizeof(abi.Integrator), C.sizeof(abi.Stats)]
-    assert offs == [abi.Scene.torus_r_max.offset, abi.Camera.fov_vert.offset, abi.Integrator.lanes_per_ray.offset,
+    assert offs == [abi.Scene.r_in_spin_slope.offset, abi.Camera.fov_vert.offset, abi.Integrator.lanes_per_ray.offset,
                     abi.Stats.lanes_per_ray.offset]
 
 
