@@ -147,7 +147,7 @@ def adjoint_check(ndraw, x_pix, first=0, edge_width=None):
         rows.append(r)
         print("  draw %d p=%s adjoint %.1fs" % (i, np.array2string(p, precision=4), r["adjoint_seconds"]) +
               "".join("\n    %-17s rel %s kept %.3f" % (t, " ".join("%.1e" % v for v in r["rel_" + t]), r["kept_" + t])
-                      for t in ("frozen", "resolved", "adaptive", "adaptive_resolved")
+                      for t in ("frozen", "resolved", "adaptive", "adaptive_resolved", "converged")
                       if "rel_" + t in r), file=sys.stderr, flush=True)
 
     def summary(key):
@@ -160,6 +160,8 @@ def adjoint_check(ndraw, x_pix, first=0, edge_width=None):
         "fd_step": "1e-4 (spin, inclination absolute; temperature, density relative)",
         "frozen_rel_err": summary("rel_frozen"), "adaptive_rel_err": summary("rel_adaptive"),
         "resolved_rel_err": summary("rel_resolved"), "adaptive_resolved_rel_err": summary("rel_adaptive_resolved"),
+        "converged_rel_err": summary("rel_converged"), "converged_fd_step": "h / 256",
+        "kept_pixels_converged_median": float(np.median([r["kept_converged"] for r in rows])),
         "kept_pixels_resolved_median": float(np.median([r["kept_resolved"] for r in rows])),
         "kept_pixels_median": float(np.median([r["kept_frozen"] for r in rows])),
         "kept_pixels_adaptive_median": float(np.median([r["kept_adaptive"] for r in rows])),
@@ -168,6 +170,7 @@ def adjoint_check(ndraw, x_pix, first=0, edge_width=None):
         "per_draw": [{"draw": first + i, "adjoint": r["adjoint_frozen"].tolist(), "fd_frozen": r["fd_frozen"].tolist(),
                       "adjoint_adaptive_set": r["adjoint_adaptive"].tolist(), "fd_adaptive": r["fd_adaptive"].tolist(),
                       "fd_resolved": r["fd_resolved"].tolist(), "adjoint_resolved_set": r["adjoint_resolved"].tolist(),
+                      "fd_converged": r["fd_converged"].tolist(), "adjoint_converged_set": r["adjoint_converged"].tolist(),
                       "kept": r["kept_frozen"], "kept_adaptive": r["kept_adaptive"], "kept_resolved": r["kept_resolved"]}
                      for i, r in enumerate(rows)]}}))
 

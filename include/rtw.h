@@ -154,13 +154,16 @@ typedef struct rtw_scene {
 
 /* Softened thick-disk edges (edge_width = w > 0; no reference counterpart, for the
  * differentiable workload).  Inside the disk
- *   rho = density (exp(-cos^2 theta / 2 h^2) - exp(-c_cut^2 / 2 h^2)) W(r),
- *   W(r) = s((r - r_in) / w) s((r_out - r) / w),  s(t) = 3t^2 - 2t^3 on [0, 1], 1 beyond,
- * with h = h_over_r and c_cut = z_cut h, so rho vanishes at all four edges and
- * rho^2 (emissivity and opacity) is C^1 across them: an image integral no longer
- * jumps when a ray or an edge moves across another.  This requires
- * r_out - r_in >= 2w.  rtw_render_adjoint differentiates W, including the inner
- * edge's motion r_in' = r_in + r_in_spin_slope (a' - a) with the spin. */
+ *   rho = density exp(-cos^2 theta / 2 h^2) s((r - r_in) / w) s((r_out - r) / w)
+ *         s((c_cut - |cos theta|) / h),
+ *   s(t) = 10t^3 - 15t^4 + 6t^5 on [0, 1], 1 beyond,
+ * with h = h_over_r and c_cut = z_cut h: the radial edges soften over w, the
+ * vertical one over the last scale height.  rho^2 (emissivity and opacity) then
+ * vanishes at every edge with its first two derivatives and is C^2 everywhere:
+ * an image integral no longer jumps when a ray or an edge moves across another.
+ * This requires r_out - r_in >= 2w and z_cut >= 1.  rtw_render_adjoint
+ * differentiates the windows, including the inner edge's motion
+ * r_in' = r_in + r_in_spin_slope (a' - a) with the spin. */
 
 /* Camera, the arguments of initialize_camera (main/initialization.jl:190-192). */
 typedef struct rtw_camera {

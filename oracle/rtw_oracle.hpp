@@ -779,13 +779,15 @@ template <class R> R freq_shift(const R* ray, const R u[4], const R g[4][4]) {
   return fs;
 }
 
-// Softened edges of the thick disk (no reference counterpart): the Gaussian less its
-// value at the vertical cut, times the radial window W(r) = s((r - r_in) / w)
-// s((r_out - r) / w) with the smoothstep s(t) = 3t^2 - 2t^3, s = 1 for t >= 1; rho
-// then vanishes at every edge and rho^2 has a continuous slope across it.
-template <class R> R edge_window(const Scene& s, R r) {
-  auto st = [](R t) { return t < R(1) ? t * t * (R(3) - R(2) * t) : R(1); };
-  return st((r - R(s.r_in)) / R(s.edge_width)) * st((R(s.r_out) - r) / R(s.edge_width));
+// Softened edges of the thick disk (no reference counterpart): the windows
+// s((r - r_in) / w) s((r_out - r) / w) s((c_cut - |cos theta|) / h) with the quintic
+// smootherstep s(t) = 6t^5 - 15t^4 + 10t^3, s = 1 for t >= 1, w = edge_width,
+// h = h_over_r, c_cut = z_cut h; rho then vanishes at every edge with two
+// derivatives and rho^2 is C^2 across it.
+template <class R> R edge_window(const Scene& s, R r, R c) {
+  auto st = [](R t) { return t < R(1) ? ((R(6) * t - R(15)) * t + R(10)) * t * t * t : R(1); };
+  return st((r - R(s.r_in)) / R(s.edge_width)) * st((R(s.r_out) - r) / R(s.edge_width)) *
+         st((R(s.z_cut * s.h_over_r) - fabs(c)) / R(s.h_over_r));
 }
 
 // Per-point fluid parameters (calc_fluid_params!, Minkowski source_model.jl:222-226:
@@ -797,8 +799,7 @@ template <class R> void fluid_params(const Scene& s, const R* x, const R u[4], c
     if (is_fire(s, x)) {
       R c = cos(x[2]);
       R rho = R(s.density) * exp(-(c * c) / R(2 * s.h_over_r * s.h_over_r));
-      if (s.edge_width > 0)
-        rho = (rho - R(s.density * std::exp(-0.5 * s.z_cut * s.z_cut))) * edge_window(s, x[1]);
+      if (s.edge_width > 0) rho *= edge_window(s, x[1], c);
       fp[1] = rho * rho;
     } else {
       fp[1] = R(0);

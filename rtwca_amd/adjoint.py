@@ -77,7 +77,7 @@ def _fd_pix(plus, minus, target, h):
 RESOLVED_TOL = 0.02
 
 
-def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, **kw):
+def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, small_div=256, **kw):
     """Adjoint of L = sum |XYZ - target|^2 against central differences (steps
     `fd_steps`), two ways:
       frozen:   differences of the recorded accepted-step map (rtw_adjoint_replay),
@@ -95,11 +95,15 @@ def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, **kw)
                 an edge band) have curvature in p far beyond the scale of h there,
                 and their central differences only approach the derivative for
                 much smaller h.
-      adaptive_resolved: the adaptive comparison over those resolved pixels.
+      adaptive_resolved: the adaptive comparison over those resolved pixels;
+      converged: (small_div > 0) the frozen map's differences at h / small_div over
+                the frozen pixels: in FP64 these are still far above rounding, and
+                they converge onto the adjoint as the step shrinks, near-critical
+                rays included (only photon-ring orbits need still smaller steps).
     Rays that took the Euler pole step, and rays whose replay or render is
     non-finite, are left out.  Returns dict with 'adjoint_<set>', 'fd_<set>',
-    'rel_<set>' and 'kept_<set>' for the sets frozen, adaptive, resolved and
-    adaptive_resolved.
+    'rel_<set>' and 'kept_<set>' for the sets frozen, adaptive, resolved,
+    adaptive_resolved and converged.
 
     Differences of renders at a much tighter tolerance are no yardstick here: the
     reference's averaged-slope FSAL makes the method low order, so a converged
@@ -114,13 +118,21 @@ def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, **kw)
     soft = kw.get("edge_width", EDGE_WIDTH) > 0
     keep_f = (f0["info"][:, 3] == 0) & np.isfinite(r0["xyz"]).all(axis=1)
     keep_a = (f0["info"][:, 3] == 0) & np.isfinite(f0["xyz"]).all(axis=1)
-    pf, mf, pa, ma, ph, mh = [], [], [], [], [], []
+    pf, mf, pa, ma, ph, mh, ps, ms = [], [], [], [], [], [], [], []
+    keep_s = keep_f.copy()
     for k in range(4):
         e = np.zeros(4)
         e[k] = h[k]
         if resolve:
             ph.append(replay(ctx, p + e / 2, x_pix, n_lambda, **kw)["xyz"])
             mh.append(replay(ctx, p - e / 2, x_pix, n_lambda, **kw)["xyz"])
+        if small_div:
+            sp = replay(ctx, p + e / small_div, x_pix, n_lambda, **kw)
+            sm = replay(ctx, p - e / small_div, x_pix, n_lambda, **kw)
+            keep_s &= (sp["info"][:, 0] == r0["info"][:, 0]) & (sm["info"][:, 0] == r0["info"][:, 0])
+            keep_s &= np.isfinite(sp["xyz"]).all(axis=1) & np.isfinite(sm["xyz"]).all(axis=1)
+            ps.append(sp["xyz"])
+            ms.append(sm["xyz"])
         rp, rm = replay(ctx, p + e, x_pix, n_lambda, **kw), replay(ctx, p - e, x_pix, n_lambda, **kw)
         # reflections, matter stage evaluations (where the horizon stop falls does not matter)
         cols = [0] if soft else [0, 2]
@@ -147,9 +159,11 @@ def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, **kw)
         keep_r = keep_f & np.isfinite(d2).all(axis=1) & ok.all(axis=1)
         sets.append(("resolved", keep_r, pf, mf))
         sets.append(("adaptive_resolved", keep_a & keep_r, pa, ma))
+    if small_div:
+        sets.append(("converged", keep_s, ps, ms))
     for tag, keep, plus, minus in sets:
         adj = g["grad_pix"][keep].sum(axis=0)
-        fd = _fd(plus, minus, target, keep, h)
+        fd = _fd(plus, minus, target, keep, h / small_div if tag == "converged" else h)
         out["adjoint_" + tag] = adj
         out["fd_" + tag] = fd
         out["rel_" + tag] = np.abs(adj - fd) / np.maximum(np.abs(fd), 1e-300)

@@ -345,7 +345,7 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
   d.inv_2h2 = 1.0 / (2 * s.h_over_r * s.h_over_r);
   d.edge_iw = s.edge_width > 0 ? 1.0 / s.edge_width : 0.0;
   d.rin_da = s.r_in_spin_slope;
-  d.g_cut = std::exp(-0.5 * s.z_cut * s.z_cut);
+  d.inv_h = 1.0 / s.h_over_r;
   d.r_escape = s.r_escape;
   d.r_photon15 = 1.5 * (3 * s.mass);
   const bool kerr_family = s.kind >= RTW_SCENE_KERR_DISK && s.kind <= RTW_SCENE_POWEHI_M87;
@@ -438,8 +438,8 @@ static int make_scene(rtw_ctx* ctx, const rtw_scene& s, const rtw_integrator& o,
     if (!ctx->gaunt_loaded) return fail(ctx, RTW_E_ARG, "free-free scene needs rtw_load_gaunt first");
     if (!(s.edge_width >= 0) || !std::isfinite(s.edge_width) || !std::isfinite(s.r_in_spin_slope))
       return fail(ctx, RTW_E_ARG, "edge_width must be finite and >= 0, r_in_spin_slope finite");
-    if (s.edge_width > 0 && !(s.r_out - s.r_in >= 2 * s.edge_width))
-      return fail(ctx, RTW_E_ARG, "softened disk edges need r_out - r_in >= 2 edge_width");
+    if (s.edge_width > 0 && !(s.r_out - s.r_in >= 2 * s.edge_width && s.z_cut >= 1))
+      return fail(ctx, RTW_E_ARG, "softened disk edges need r_out - r_in >= 2 edge_width and z_cut >= 1");
     d.ff_C = s.absorb * std::sqrt(FF_T0 / s.temperature);
     // Gaunt column position for this temperature, Float32 as emission_functions.jl:58-66
     double gam2 = GAM2_FACTOR / s.temperature;

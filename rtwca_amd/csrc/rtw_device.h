@@ -46,7 +46,7 @@ struct DevScene {
   double cos_cut;      // thick disk: |cos theta| <= z_cut * H/r
   double inv_2h2;      // 1 / (2 (H/r)^2)
   double edge_iw;      // thick disk: 1 / radial edge width (0: hard edges)
-  double g_cut;        // thick disk with softened edges: the Gaussian at the vertical cut
+  double inv_h;        // thick disk: 1 / (H/r), the vertical window's width in cos theta
   double rin_da;       // thick disk: d r_in / d a of the softened inner edge
   double r_escape;
   double r_horizon;    // (1 + 10 rtol) r_ev; 0 without horizon
@@ -652,16 +652,20 @@ __device__ __forceinline__ void kerr_rhs(const DevScene& S, const T& a, const T 
       if (m) {
         T rho = T(S.density) * exp_(-(c * c) * T(S.inv_2h2));
         if (S.edge_iw > 0) {
-          // softened edges: the Gaussian less its value at the vertical cut, times
-          // W = s((r - r_in) / w) s((r_out - r) / w), s(t) = 3t^2 - 2t^3 up to t = 1; the
+          // softened edges: windows s(t) = t^3 (10 - 15t + 6t^2) up to t = 1 over w at
+          // both radial edges and over one scale height below the vertical cut; the
           // inner edge moves with the spin as r_in + (a - a_scene) dr_in/da (the shift
           // is zero here except in the dual-number adjoint)
           const T iw = T(S.edge_iw);
           const T rin = T(S.r_in) + (a - T(S.a)) * T(S.rin_da);
-          const T ti = (r - rin) * iw, to = (T(S.r_out) - r) * iw;
-          const T si = (ti < T(1)) ? ti * ti * (T(3) - T(2) * ti) : T(1);
-          const T so = (to < T(1)) ? to * to * (T(3) - T(2) * to) : T(1);
-          rho = (rho - T(S.density * S.g_cut)) * (si * so);
+          const T t3[3] = {(r - rin) * iw, (T(S.r_out) - r) * iw, (T(S.cos_cut) - fabs(c)) * T(S.inv_h)};
+          T win = T(1);
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            const T t = t3[e];
+            if (t < T(1)) win = win * (t * t * t * (T(10) + t * (T(6) * t - T(15))));
+          }
+          rho = rho * win;
         }
         o.w = rho * rho;
       }

@@ -143,9 +143,10 @@ def kerr_thick_ff(spin=0.9, temperature=1e4, density=1.0, kappa=0.2, r_in=None, 
     """Kerr geometry (main/Kerr.source/source_geometry.jl in FP64) with a Gaussian thick
     disk rho = rho_s exp(-z^2 / 2H^2), H = 0.1 r, |z| <= 4H, radiating free-free with the
     van Hoof Gaunt factor (main/emission_functions.jl:54-78) and Kirchhoff's law.
-    edge_width > 0 softens the edges (the Gaussian less its value at the cut, radial
-    smoothstep windows of that width); with the default inner edge (the
-    reference-formula ISCO of the spin) the scene also carries d r_in / d spin."""
+    edge_width > 0 softens the edges (quintic smootherstep windows of that width at
+    the radial edges and of one scale height below the vertical cut); with the
+    default inner edge (the reference-formula ISCO of the spin) the scene also
+    carries d r_in / d spin."""
     s = _base(abi.SCENE_KERR_THICK_FF)
     s.spin = spin
     s.temperature = temperature

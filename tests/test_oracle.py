@@ -21,10 +21,9 @@ plausible transcription mistake would break:
   camera (initialization.jl:190)      null initial momenta; pixel angle closed form
   keepinbounds/pad_max_dt/terminate   constructed states with known answers
   ISCO quirk, circular-orbit beta     closed forms
-  softened thick-disk edges           zero at every edge with the window's order (r^4
-                                      radial, z^2 vertical), smoothstep symmetry
-                                      s(t) + s(1 - t) = 1, hard disk less the cut value
-                                      inside; ISCO spin slope against differences
+  softened thick-disk edges           rho^2 of sixth order at every edge, window symmetry
+                                      s(t) + s(1 - t) = 1 and integral 1/2, the hard
+                                      disk inside; ISCO spin slope against differences
 Parity is unpinned (documented, not tested) for: Julia's libm last-ulp behaviour,
 Colors.jl's exact RGB{N0f16} rounding, and the order of 4x4 matrix products in the
 camera set-up -- all ulp-level.
@@ -482,39 +481,40 @@ def _rho2(s, r, th):
 
 
 def test_softened_disk_edges():
-    """The softened thick disk (edge_width > 0): rho vanishes at the four edges and
-    rho^2 with it to the orders of the windows (quartic at the radial smoothstep
-    edges, quadratic at the shifted Gaussian's cut); s(t) + s(1 - t) = 1 across a
-    radial band; well inside, rho is the hard disk's rho less rho_s exp(-z_cut^2 / 2)."""
+    """The softened thick disk (edge_width > 0): rho vanishes at the four edges with
+    two derivatives, so rho^2 grows as the sixth power of the distance into the
+    disk (radially and vertically); s(t) + s(1 - t) = 1 across a band; well inside,
+    rho is the hard disk's rho."""
     w, rho_s = 1.0, 3.0
     soft = scenes.kerr_thick_ff(spin=0.9, density=rho_s, edge_width=w)
     hard = scenes.kerr_thick_ff(spin=0.9, density=rho_s)
     rin, rout = soft.r_in, soft.r_out
     th0 = PI / 2 - 0.03
-    # interior: the Gaussian less its value at the cut
     for r in (rin + 1.5 * w, 8.0, rout - 1.2 * w):
-        want = math.sqrt(_rho2(hard, r, th0)) - rho_s * math.exp(-8.0)
-        assert math.sqrt(_rho2(soft, r, th0)) == pytest.approx(want, rel=1e-13)
+        assert _rho2(soft, r, th0) == pytest.approx(_rho2(hard, r, th0), rel=1e-14)
     interior = math.sqrt(_rho2(soft, 8.0, th0))
-    # radial bands: quartic onset, symmetric smoothstep, zero outside
-    for e in (1e-3, 2e-3):
-        assert _rho2(soft, rin + e, th0) > 0
-    assert _rho2(soft, rin + 2e-3, th0) / _rho2(soft, rin + 1e-3, th0) == pytest.approx(16.0, rel=1e-2)
-    assert _rho2(soft, rout - 2e-3, th0) / _rho2(soft, rout - 1e-3, th0) == pytest.approx(16.0, rel=1e-2)
+    # radial bands: sixth-power onset, symmetric window, zero outside
+    assert _rho2(soft, rin + 1e-3, th0) > 0
+    assert _rho2(soft, rin + 2e-3, th0) / _rho2(soft, rin + 1e-3, th0) == pytest.approx(64.0, rel=1e-2)
+    assert _rho2(soft, rout - 2e-3, th0) / _rho2(soft, rout - 1e-3, th0) == pytest.approx(64.0, rel=1e-2)
     assert _rho2(soft, rin - 1e-9, th0) == 0 and _rho2(soft, rout + 1e-9, th0) == 0
     for x in (0.1, 0.37, 0.5):
         # the Gaussian factor is the same at one theta, so W(r_in + x) + W(r_in + w - x) = 1
         ratio = (math.sqrt(_rho2(soft, rin + x * w, th0)) + math.sqrt(_rho2(soft, rin + (1 - x) * w, th0))) / interior
         assert ratio == pytest.approx(1.0, rel=1e-12)
-        s_t = math.sqrt(_rho2(soft, rout - x * w, th0)) / interior
-        assert s_t == pytest.approx(3 * x * x - 2 * x ** 3, rel=1e-12)
-    # vertical cut |cos th| = z_cut h: rho linear in the distance to it, so rho^2 quadratic
-    c_cut = soft.z_cut * soft.h_over_r
-    th_cut = math.acos(c_cut)
+    # the window's integral over its band is 1/2 (trapezoid sum over 2000 cells)
+    xs = np.linspace(0.0, 1.0, 2001)
+    vals = np.array([math.sqrt(_rho2(soft, rout - x * w, th0)) / interior for x in xs])
+    assert ((vals[1:] + vals[:-1]) / 2).mean() == pytest.approx(0.5, abs=1e-6)
+    # vertical: the window over the last scale height below |cos th| = z_cut h
+    h = soft.h_over_r
+    th_cut = math.acos(soft.z_cut * h)
+    g = lambda th: math.exp(-math.cos(th) ** 2 / (2 * h * h))
     q1, q2 = _rho2(soft, 8.0, th_cut + 1e-4), _rho2(soft, 8.0, th_cut + 2e-4)
-    assert q1 > 0 and q2 / q1 == pytest.approx(4.0, rel=1e-2)
+    assert q1 > 0 and q2 / q1 == pytest.approx(64.0 * (g(th_cut + 2e-4) / g(th_cut + 1e-4)) ** 2, rel=1e-2)
     assert _rho2(soft, 8.0, th_cut - 1e-9) == 0
-    assert _rho2(hard, 8.0, th_cut + 1e-9) / _rho2(hard, 8.0, PI / 2) == pytest.approx(math.exp(-16), rel=1e-6)
+    th_in = math.acos((soft.z_cut - 1.0) * h) + 1e-9  # one scale height inside: window 1
+    assert _rho2(soft, 8.0, th_in) == pytest.approx(_rho2(hard, 8.0, th_in), rel=1e-12)
     # edge_width 0 is the hard disk
     assert _rho2(scenes.kerr_thick_ff(spin=0.9, density=rho_s, edge_width=0.0), 8.0, th0) == _rho2(hard, 8.0, th0)
 
