@@ -86,6 +86,9 @@ def parse():
                          "gradients vs central finite differences, and exit")
     ap.add_argument("--adjoint-x-pix", type=int, default=256)
     ap.add_argument("--adjoint-first", type=int, default=0, help="index of the first draw of --adjoint-check")
+    ap.add_argument("--adjoint-convergence", type=str, default="",
+                    help="comma-separated draw indices of --adjoint-check: spin and inclination gradients against "
+                         "frozen-map central differences at h = 1e-4 / 16^k, k = 0..4, and exit")
     ap.add_argument("--adjoint-edge-width", type=float, default=None,
                     help="disk edge width of --adjoint-check (default: the workload's softened edges; 0: hard)")
     return ap.parse_args()
@@ -173,6 +176,40 @@ def adjoint_check(ndraw, x_pix, first=0, edge_width=None):
                       "fd_converged": r["fd_converged"].tolist(), "adjoint_converged_set": r["adjoint_converged"].tolist(),
                       "kept": r["kept_frozen"], "kept_adaptive": r["kept_adaptive"], "kept_resolved": r["kept_resolved"]}
                      for i, r in enumerate(rows)]}}))
+
+
+def adjoint_convergence(draws, x_pix):
+    """For the given draws of --adjoint-check (same seeds, softened edges): the spin and
+    inclination gradients of the L2 loss against central differences of the recorded
+    step map at h = 1e-4 / 16^k, k = 0..4, over every pixel (Euler-step rays, non-finite
+    replays and reflections moving to another step aside).  Prints one JSON line."""
+    from rtwca_amd import adjoint, host
+    ctx = host.Context(0)
+    out = []
+    for d in draws:
+        rng = np.random.default_rng([20240601, d])
+        p = adjoint.draw(rng)
+        ptgt = p * (1 + 0.05 * rng.standard_normal(4))
+        tgt = adjoint.render(ctx, ptgt, x_pix, 64)["xyz"]
+        f0 = adjoint.render(ctx, p, x_pix, 64)
+        g = adjoint.render(ctx, p, x_pix, 64, dL_dxyz=2.0 * (f0["xyz"] - tgt), want_grad_pix=True)
+        r0 = adjoint.replay(ctx, p, x_pix, 64)
+        for k in (0, 1):
+            for j in range(5):
+                h = 1e-4 / 16 ** j
+                e = np.zeros(4)
+                e[k] = h
+                rp, rm = adjoint.replay(ctx, p + e, x_pix, 64), adjoint.replay(ctx, p - e, x_pix, 64)
+                ok = (f0["info"][:, 3] == 0) & np.isfinite(rp["xyz"]).all(1) & np.isfinite(rm["xyz"]).all(1)
+                ok &= (rp["info"][:, 0] == r0["info"][:, 0]) & (rm["info"][:, 0] == r0["info"][:, 0])
+                fd = (((rp["xyz"][ok] - tgt[ok]) ** 2).sum() - ((rm["xyz"][ok] - tgt[ok]) ** 2).sum()) / (2 * h)
+                adj = float(g["grad_pix"][ok, k].sum())
+                row = {"draw": d, "param": adjoint.PARAMS[k], "h": h, "fd": float(fd), "adjoint": adj,
+                       "rel": abs(adj - fd) / abs(fd), "kept": float(ok.mean())}
+                print("  draw %d %-11s h %.2e fd % .8e adjoint % .8e rel %.1e" %
+                      (d, row["param"], h, fd, adj, row["rel"]), file=sys.stderr, flush=True)
+                out.append(row)
+    print(json.dumps({"adjoint_convergence": {"workload": "kerr_thick_ff_adjoint_%d" % x_pix, "rows": out}}))
 
 
 def gpu_render(ctx, w, precision, steps, warmup, torch, schedule=None):
@@ -323,6 +360,9 @@ def main():
         raise SystemExit("bench.py needs a GPU")
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib, torch)
+        return
+    if a.adjoint_convergence:
+        adjoint_convergence([int(v) for v in a.adjoint_convergence.split(",")], a.adjoint_x_pix)
         return
     if a.adjoint_check:
         adjoint_check(a.adjoint_check, a.adjoint_x_pix, a.adjoint_first, a.adjoint_edge_width)
