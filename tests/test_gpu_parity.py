@@ -195,3 +195,28 @@ def test_per_kernel_stats(ctx):
     ctx.render(w2.scene, w2.cameras, w2.integ, w2.lam)
     s2 = ctx.last_stats()
     assert s2["steps_group"] == s2["steps_attempted"] and s2["group_launches"] == 1 and s2["vacuum_launches"] == 0
+
+
+def test_softened_disk_mixed_vs_fp64_and_oracle(ctx):
+    """The softened thick disk (edge windows in the FP32 stages too): the FP64 kernel
+    against the oracle and the mixed kernel against the FP64 kernel, to the bars
+    above; the windows dim the disk near its edges only."""
+    import oracle as O
+    w = scenes.config(3, x_pix=16, n_lambda=64)
+    soft = scenes.kerr_thick_ff(spin=0.9, edge_width=1.0)
+    g64 = ctx.render(soft, w.cameras, w.integ, w.lam)
+    o = O.render(soft, w.cameras[0], w.integ, w.lam, nthreads=8)
+    _check(g64, o["xyz"], o["cls"], 1e-5, cls_bar=0.99)
+    w.integ.precision = abi.PREC_MIXED
+    gm = ctx.render(soft, w.cameras, w.integ, w.lam)
+    _check(gm, g64["xyz"], g64["cls"], 1e-3, cls_bar=0.99)
+    hard = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    assert np.all(gm["xyz"][:, 1] <= hard["xyz"][:, 1] * (1 + 1e-2) + 1e-30)
+
+
+def test_softened_disk_argument_errors(ctx):
+    w = scenes.config(3, x_pix=8, n_lambda=8)
+    for kw in ({"edge_width": -1.0}, {"edge_width": 5.0}, {"edge_width": 1.0, "z_cut": 0.5},
+               {"edge_width": float("nan")}):
+        with pytest.raises(host.RTWError, match="edge"):
+            ctx.render(scenes.kerr_thick_ff(spin=0.9, **kw), w.cameras, w.integ, w.lam)
