@@ -77,6 +77,21 @@ def test_adjoint_matches_frozen_and_adaptive_differences(ctx):
     assert np.max(r["rel_frozen"]) <= 2e-2 and np.max(r["rel_adaptive"]) <= 2e-2, r
 
 
+def test_adjoint_at_a_corner_of_the_parameter_box(ctx):
+    """Spin 0.95 (inner edge at its smallest radius and fastest motion with the spin),
+    edge-on camera (85 degrees), the hottest and thinnest disk: temperature and
+    density match the frozen map's differences to 1e-5 and every parameter
+    matches them at h / 256 to 1e-3, over every pixel."""
+    p = np.array([0.95, np.radians(85.0), 2.0e4, 0.1])
+    rng = np.random.default_rng(7)
+    tgt = adjoint.render(ctx, p * (1 + 0.05 * rng.standard_normal(4)), 24, 64)["xyz"]
+    r = adjoint.gradcheck(ctx, p, tgt, 24, 64)
+    _report(r)
+    assert r["kept_frozen"] > 0.5 and r["kept_converged"] > 0.9
+    assert np.max(r["rel_frozen"][2:]) <= 1e-5, r
+    assert np.max(r["rel_converged"]) <= 1e-3, r
+
+
 def test_adjoint_hard_edges(ctx):
     """Hard disk edges: the pathwise adjoint holds the matter indicator fixed, so
     the comparisons keep only pixels whose stage points cross no edge within +-h."""
