@@ -220,3 +220,25 @@ def test_softened_disk_argument_errors(ctx):
                {"edge_width": float("nan")}):
         with pytest.raises(host.RTWError, match="edge"):
             ctx.render(scenes.kerr_thick_ff(spin=0.9, **kw), w.cameras, w.integ, w.lam)
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHED_GRID, abi.SCHED_PERSISTENT, abi.SCHED_WAVEFRONT])
+def test_zero_colours_on_a_ragged_grid(ctx, schedule):
+    """No wavelengths on a 7 x 3 grid, fewer rays than one lane-group block.  The
+    reference's opacity stop, every tau_c >= -log(atol) (calc_terminate, e.g.
+    Kerr.source/source_model.jl:168-182), holds vacuously for no colours, so every
+    ray stops after its first step; classes and step counts equal the oracle's, XYZ
+    is zero, in FP64 and mixed precision and under every schedule."""
+    import oracle as O
+    w = scenes.config(3, x_pix=7, n_lambda=4)
+    cam = abi.make_camera([0, 5, np.pi / 2, 0], [0, 0], scenes.kerr_circ_beta(5.0, 0.9), [-np.pi / 2, 0, 0], 7, 90, 50)
+    w.integ.schedule = schedule
+    g = ctx.render(w.scene, [cam], w.integ, [])
+    o = O.render(w.scene, cam, w.integ, [])
+    assert g["y_pix"] == 3 and len(g["cls"]) == 21
+    assert np.array_equal(g["cls"], o["cls"])
+    assert np.array_equal(g["steps"], o["steps"]) and np.all(o["steps"][:, 0] == 1)
+    assert not g["xyz"].any() and not o["xyz"].any()
+    w.integ.precision = abi.PREC_MIXED
+    gm = ctx.render(w.scene, [cam], w.integ, [])
+    assert np.array_equal(gm["cls"], g["cls"]) and np.array_equal(gm["steps"], g["steps"]) and not gm["xyz"].any()
