@@ -242,3 +242,20 @@ def test_zero_colours_on_a_ragged_grid(ctx, schedule):
     w.integ.precision = abi.PREC_MIXED
     gm = ctx.render(w.scene, [cam], w.integ, [])
     assert np.array_equal(gm["cls"], g["cls"]) and np.array_equal(gm["steps"], g["steps"]) and not gm["xyz"].any()
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHED_GRID, abi.SCHED_WAVEFRONT])
+def test_step_limit(ctx, schedule):
+    """max_steps = 600 (calc_terminate's step_count >= max_steps): the rays still
+    running there (about half, the ones through the disk) stop with class BOUNDARY
+    after exactly 600 attempts, as in the oracle; the others fall into the hole."""
+    import oracle as O
+    w = scenes.config(3, x_pix=12, n_lambda=16, schedule=schedule)
+    w.integ.max_steps = 600
+    g = ctx.render(w.scene, w.cameras, w.integ, w.lam)
+    o = O.render(w.scene, w.cameras[0], w.integ, w.lam, nthreads=8)
+    capped = o["steps"][:, 0] == 600
+    assert 0.3 < capped.mean() < 0.9 and np.all(o["cls"][capped] == abi.CLASS_BOUNDARY)
+    assert np.all(g["steps"][:, 0] <= 600)
+    _check(g, o["xyz"], o["cls"], 1e-5, cls_bar=0.99)
+    assert np.mean(g["steps"][:, 0] == o["steps"][:, 0]) >= 0.95
