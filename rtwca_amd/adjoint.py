@@ -120,6 +120,8 @@ def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, small
     keep_a = (f0["info"][:, 3] == 0) & np.isfinite(f0["xyz"]).all(axis=1)
     pf, mf, pa, ma, ph, mh, ps, ms = [], [], [], [], [], [], [], []
     keep_s = keep_f.copy()
+    # reflections, matter stage evaluations (where the horizon stop falls does not matter)
+    cols = [0] if soft else [0, 2]
     for k in range(4):
         e = np.zeros(4)
         e[k] = h[k]
@@ -129,13 +131,12 @@ def gradcheck(ctx, p, target, x_pix=64, n_lambda=64, h=None, resolve=True, small
         if small_div:
             sp = replay(ctx, p + e / small_div, x_pix, n_lambda, **kw)
             sm = replay(ctx, p - e / small_div, x_pix, n_lambda, **kw)
-            keep_s &= (sp["info"][:, 0] == r0["info"][:, 0]) & (sm["info"][:, 0] == r0["info"][:, 0])
+            keep_s &= (sp["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
+            keep_s &= (sm["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
             keep_s &= np.isfinite(sp["xyz"]).all(axis=1) & np.isfinite(sm["xyz"]).all(axis=1)
             ps.append(sp["xyz"])
             ms.append(sm["xyz"])
         rp, rm = replay(ctx, p + e, x_pix, n_lambda, **kw), replay(ctx, p - e, x_pix, n_lambda, **kw)
-        # reflections, matter stage evaluations (where the horizon stop falls does not matter)
-        cols = [0] if soft else [0, 2]
         keep_f &= (rp["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
         keep_f &= (rm["info"][:, cols] == r0["info"][:, cols]).all(axis=1)
         keep_f &= np.isfinite(rp["xyz"]).all(axis=1) & np.isfinite(rm["xyz"]).all(axis=1)
